@@ -1,0 +1,181 @@
+"""Benchmark of the D3Q19 ComputeUnit main loop on one MI355X.
+
+Workload (the reference's GPU configuration): a Rushton-turbine tank on a 512^3
+grid (T = 512, impeller D = T/3, 6 blades, clearance T/3, tip speed 0.1,
+Re = 1e5 with nu from calcNuAndRe_m), FP32 storage and arithmetic,
+Eggels-Somers collision with Smagorinsky LES (cs0 = 0.1), fluid initialised at
+rest (rho = 1).  One step is one iteration of the reference main loop
+(tdlbcpp/src/main.cpp:336-455): blade rotation, collision, bounce-back,
+streaming, moments and IB forcing, all on the device.
+
+Every streaming x layout variant is run; the headline value is Esotwist/LIJK.
+value = interior cells x steps / seconds / 1e6 (MLUPS), timed with HIP events
+on the ComputeUnit's stream around exactly --steps steps after --warmup steps.
+The roofline entry uses the stream-collide kernel's own event-timed average
+duration and its algorithmic bytes (2 x 18 x 4 B populations + 2 x 4 B Nu per
+cell).  cpu_baseline times the CPU oracle (a single-threaded restatement of the
+reference's CPU path) on a bounded sample: the same tank at 96^3 for a few steps.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS & %HBM roofline, FP32 vs FP16-store; p50/p99 latency of batched tanks"
+HBM_PEAK_GBS = 8000.0
+
+
+class Hip:
+    """HIP events on a given stream (libamdhip64 through ctypes)."""
+
+    def __init__(self):
+        self.L = ctypes.CDLL("libamdhip64.so")
+        self.L.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.L.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.L.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.L.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def event(self):
+        e = ctypes.c_void_p()
+        assert self.L.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.L.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.L.hipEventSynchronize(b) == 0
+        ms = ctypes.c_float()
+        assert self.L.hipEventElapsedTime(ctypes.byref(ms), a, b) == 0
+        return ms.value
+
+    def device_sync(self):
+        assert self.L.hipDeviceSynchronize() == 0
+
+
+def bytes_per_cell(prec_bytes=4, les=True):
+    return 2 * 18 * prec_bytes + (2 * prec_bytes if les else 0)
+
+
+def run_variant(td, hip, grid, streaming, layout, steps, warmup, re=1.0e5, uav=0.1):
+    flow = td.flow_for_tank(grid, re=re, uav=uav, cs0=0.1, use_les=1)
+    cu = td.ComputeUnit(grid, layout=layout, streaming=streaming, precision=td.PREC_F32, **flow)
+    rp = td.rushton_defaults(grid, tip_speed=uav, startup_steps=30)
+    t0 = time.time()
+    cu.initialise(1.0)
+    cu.bind_rushton(rp, step=1, angle=0.0)
+    cu.synchronize()
+    setup_s = time.time() - t0
+    cu.run(warmup)
+    cu.synchronize()
+    s = cu.stream()
+    e0, e1 = hip.event(), hip.event()
+    cu.profile(True)
+    hip.record(e0, s)
+    cu.run(steps)
+    hip.record(e1, s)
+    total_ms = hip.elapsed_ms(e0, e1)
+    coll_ms, ncoll, force_ms, nforce = cu.profile_read()
+    cu.profile(False)
+    n_static, n_rot, n_levels = cu.forcing_info()
+    cells = grid ** 3
+    res = dict(mlups=cells * steps / (total_ms / 1e3) / 1e6, ms_per_step=total_ms / steps,
+               collide_ms=coll_ms / max(ncoll, 1), forcing_ms=force_ms / max(nforce, 1),
+               ib_points_static=n_static, ib_points_rotating=n_rot, forcing_levels=n_levels,
+               setup_s=round(setup_s, 2))
+    res["collide_gbs"] = bytes_per_cell() * cells / (res["collide_ms"] / 1e3) / 1e9
+    cu.close()
+    return res
+
+
+def cpu_baseline(sample_grid=96, max_seconds=20.0):
+    """The oracle (single-threaded CPU restatement of the reference path) on the same
+    tank at sample_grid^3, Simple streaming, IJKL, FP32, LES: the configuration
+    main.cpp runs on the CPU."""
+    import numpy as np
+    import tdlbcpp_amd as td
+    from oracle import oracle as O
+    from tests.helpers import ReferenceLoop, flow_for_tank
+    oflow, _ = flow_for_tank(sample_grid, re=7300.0)
+    o = O.OracleCU(sample_grid, dtype=O.F32, layout=O.IJKL, streaming=O.SIMPLE, use_les=True, **oflow)
+    rp = td.rushton_defaults(sample_grid)
+    o.initialise(1.0)
+    loop = ReferenceLoop(o, rp)
+    loop.run(1)  # warm caches
+    steps, t0 = 0, time.time()
+    while True:
+        loop.run(1)
+        steps += 1
+        el = time.time() - t0
+        if el > max_seconds or steps >= 50:
+            break
+    del np
+    return dict(value=round(sample_grid ** 3 * steps / el / 1e6, 3), unit="MLUPS", cores=1, kind="port",
+                sample=f"oracle/liboracle.so: Rushton tank {sample_grid}^3, Simple/IJKL, FP32, LES, "
+                       f"{steps} main-loop steps in {el:.1f} s on one host core")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--variants", default="all", help="all or headline")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs one ComputeUnit on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU slab decomposition is designed, not built: see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import tdlbcpp_amd as td
+    hip = Hip()
+    variants = [("Esotwist", "LIJK", td.STREAM_ESOTWIST, td.LAYOUT_LIJK)]
+    if args.variants == "all":
+        variants += [("Esotwist", "IJKL", td.STREAM_ESOTWIST, td.LAYOUT_IJKL),
+                     ("Simple", "LIJK", td.STREAM_SIMPLE, td.LAYOUT_LIJK),
+                     ("Simple", "IJKL", td.STREAM_SIMPLE, td.LAYOUT_IJKL)]
+    results = {}
+    for name, lay, st, ly in variants:
+        r = run_variant(td, hip, args.grid, st, ly, args.steps, args.warmup)
+        results[f"{name}/{lay}"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+        print(f"[bench] {name}/{lay}: {r['mlups']:.0f} MLUPS, {r['ms_per_step']:.3f} ms/step, "
+              f"collide {r['collide_ms']:.3f} ms ({r['collide_gbs']:.0f} GB/s), forcing {r['forcing_ms']:.3f} ms",
+              file=sys.stderr, flush=True)
+    head = results["Esotwist/LIJK"]
+    achieved = head["collide_gbs"]
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_esotwist_lijk_512.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC, "value": round(head["mlups"], 1), "unit": "MLUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: Rushton tank at rest, geometry from this library's generator",
+        "config": {"workload": f"rushton_tank_{args.grid}cubed_esotwist_lijk_fp32_les", "grid": args.grid,
+                   "streaming": "Esotwist", "layout": "LIJK", "precision": "fp32", "les_cs0": 0.1,
+                   "reynolds": 1.0e5, "tip_speed": 0.1, "variants": results},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_esotwist<float,LIJK,FUSED>",
+                     "bytes_per_cell": bytes_per_cell(), "cells_per_launch": args.grid ** 3},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(max_seconds=args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,345 @@
+"""Python binding of the MI355X ComputeUnit C ABI (include/tdlb.h, libtdlb_hip.so).
+
+The host-side product code is C++ (the C ABI and the tdlb_run driver); this
+module is the thin ctypes layer the tests and bench.py drive it through.
+Method names follow the reference's ComputeUnitBase (tdlbcpp/src/ComputeUnit.h:97-201).
+The library must be built (``make -C tdlbcpp_amd``); loading fails loudly if it
+is missing, there is no fallback implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtdlb_hip.so")
+
+LAYOUT_IJKL, LAYOUT_LIJK = 0, 1
+STREAM_SIMPLE, STREAM_ESOTWIST = 0, 1
+PREC_F32, PREC_F64 = 0, 1
+FIELD_Q, FIELD_F, FIELD_NU, FIELD_O = 0, 1, 2, 3
+GEOM_FIXED, GEOM_ROTATING_NON_UPDATING, GEOM_ROTATING = 0, 1, 2
+
+POINT_DTYPE = np.dtype([("i", "<i8"), ("j", "<i8"), ("k", "<i8"), ("i_frac", "<f8"), ("k_frac", "<f8"),
+                        ("u_delta", "<f8"), ("v_delta", "<f8"), ("w_delta", "<f8")])
+
+
+class CuParams(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int64), ("y", ctypes.c_int64), ("z", ctypes.c_int64), ("ghost", ctypes.c_int64),
+                ("i0", ctypes.c_int64), ("j0", ctypes.c_int64), ("k0", ctypes.c_int64),
+                ("resolution", ctypes.c_int64),
+                ("idi", ctypes.c_int32), ("idj", ctypes.c_int32), ("idk", ctypes.c_int32),
+                ("node_id", ctypes.c_int32), ("device_id", ctypes.c_int32),
+                ("layout", ctypes.c_int32), ("streaming", ctypes.c_int32), ("precision", ctypes.c_int32)]
+
+
+class Flow(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("initial_rho", "re_m_nondimensional", "uav", "cs0", "g3", "nu",
+                                               "fx0", "fy0", "fz0", "re_m", "re_f", "uf", "alpha", "beta")] + \
+               [("use_les", ctypes.c_int32), ("collision", ctypes.c_int32)]
+
+
+class RushtonParams(ctypes.Structure):
+    _fields_ = [("tank_diameter", ctypes.c_int64), ("height", ctypes.c_int64),
+                ("center_x", ctypes.c_double), ("center_z", ctypes.c_double), ("tank_radius", ctypes.c_double),
+                ("n_baffles", ctypes.c_int32), ("baffle_width", ctypes.c_double),
+                ("baffle_angle0", ctypes.c_double), ("disk_plane", ctypes.c_int64),
+                ("disk_radius", ctypes.c_double), ("hub_radius", ctypes.c_double),
+                ("shaft_radius", ctypes.c_double), ("n_blades", ctypes.c_int32),
+                ("blade_outer_radius", ctypes.c_double), ("blade_inner_radius", ctypes.c_double),
+                ("blade_plane_lo", ctypes.c_int64), ("blade_planes", ctypes.c_int64),
+                ("tip_speed", ctypes.c_double), ("startup_steps", ctypes.c_int64)]
+
+
+# FlowParams<T> defaults (tdlbcpp/src/Params/FlowParams.hpp:27-61)
+FLOW_DEFAULTS = dict(initial_rho=8.0, re_m_nondimensional=7300.0, uav=0.1, cs0=0.12, g3=0.8, nu=0.0,
+                     fx0=0.0, fy0=0.0, fz0=0.0, re_m=0.0, re_f=0.0, uf=0.0, alpha=0.97, beta=1.9,
+                     use_les=0, collision=0)
+
+
+class TdlbError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libtdlb_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TdlbError(f"{LIB_PATH} is missing: build it with `make -C {HERE}` "
+                            "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64, u64, d, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double, ctypes.c_int
+        P = ctypes.POINTER
+        sig = {
+            "tdlb_last_error": ([], ctypes.c_char_p),
+            "tdlb_version": ([], i32),
+            "tdlb_device_count": ([P(i32)], i32),
+            "tdlb_flow_calc_nu_re_m": ([P(Flow), ctypes.c_int32, ctypes.c_int32], i32),
+            "tdlb_cu_create": ([P(CuParams), P(Flow), P(vp)], i32),
+            "tdlb_cu_destroy": ([vp], i32),
+            "tdlb_cu_set_flow": ([vp, P(Flow)], i32),
+            "tdlb_cu_initialise": ([vp, d], i32),
+            "tdlb_cu_collision": ([vp], i32),
+            "tdlb_cu_moments": ([vp], i32),
+            "tdlb_cu_stream_push": ([vp], i32),
+            "tdlb_cu_stream_pull": ([vp], i32),
+            "tdlb_cu_bounce_back": ([vp], i32),
+            "tdlb_cu_forcing": ([vp, vp, i64, d, d], i32),
+            "tdlb_cu_field_bytes": ([vp, i32, P(ctypes.c_size_t)], i32),
+            "tdlb_cu_download": ([vp, i32, vp, ctypes.c_size_t], i32),
+            "tdlb_cu_upload": ([vp, i32, vp, ctypes.c_size_t], i32),
+            "tdlb_cu_get_even_step": ([vp, P(i32)], i32),
+            "tdlb_cu_set_even_step": ([vp, i32], i32),
+            "tdlb_cu_dims": ([vp, P(i64), P(i64), P(i64), P(i64)], i32),
+            "tdlb_cu_synchronize": ([vp], i32),
+            "tdlb_cu_stream": ([vp, P(vp)], i32),
+            "tdlb_rushton_defaults": ([i64, i64, d, i64, P(RushtonParams)], i32),
+            "tdlb_rushton_generate": ([P(RushtonParams), i32, d, d, vp, i64, P(i64)], i32),
+            "tdlb_rushton_step_increment": ([P(RushtonParams), u64, P(d)], i32),
+            "tdlb_rushton_solid_mask": ([P(RushtonParams), i64, i64, i64, i64, vp], i32),
+            "tdlb_cu_bind_rushton": ([vp, P(RushtonParams), u64, d, d, d], i32),
+            "tdlb_cu_run": ([vp, i64, P(u64), P(d)], i32),
+            "tdlb_cu_rotating_points": ([vp, vp, i64, P(i64)], i32),
+            "tdlb_cu_solid_mask": ([vp, vp, ctypes.c_size_t], i32),
+            "tdlb_cu_forcing_info": ([vp, P(i64), P(i64), P(i64)], i32),
+            "tdlb_cu_profile": ([vp, i32], i32),
+            "tdlb_cu_profile_read": ([vp, P(d), P(i64), P(d), P(i64)], i32),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names declared in include/tdlb.h that the library must export."""
+    hdr = os.path.join(os.path.dirname(HERE), "include", "tdlb.h")
+    import re
+    names = re.findall(r"^(?:int|const char\*)\s+(tdlb_\w+)\s*\(", open(hdr).read(), flags=re.M)
+    return names
+
+
+def check(rc):
+    if rc != 0:
+        raise TdlbError(f"tdlb error {rc}: {lib().tdlb_last_error().decode()}")
+
+
+def device_count():
+    n = ctypes.c_int()
+    check(lib().tdlb_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def make_flow(**kw):
+    vals = dict(FLOW_DEFAULTS)
+    vals.update(kw)
+    f = Flow()
+    for k, v in vals.items():
+        setattr(f, k, v)
+    return f
+
+
+def flow_for_tank(tank_diameter, re=7300.0, uav=0.1, cs0=0.1, use_les=1, precision=PREC_F32, **kw):
+    """FlowParams as main.cpp:274-276 leaves them: nu and Re_m from calcNuAndRe_m
+    with the integer impeller blade radius int(T/6)."""
+    vals = dict(initial_rho=1.0, re_m_nondimensional=re, uav=uav, cs0=cs0, use_les=use_les)
+    vals.update(kw)
+    f = make_flow(**vals)
+    check(lib().tdlb_flow_calc_nu_re_m(ctypes.byref(f), int(tank_diameter / 6), precision))
+    return {n: getattr(f, n) for n, _ in Flow._fields_}
+
+
+def rushton_defaults(tank_diameter, height=None, tip_speed=0.1, startup_steps=30):
+    p = RushtonParams()
+    check(lib().tdlb_rushton_defaults(tank_diameter, height or tank_diameter, tip_speed, startup_steps,
+                                      ctypes.byref(p)))
+    return p
+
+
+def rushton_generate(p, group, angle=0.0, delta_angle=0.0):
+    n = ctypes.c_int64()
+    check(lib().tdlb_rushton_generate(ctypes.byref(p), group, angle, delta_angle, None, 0, ctypes.byref(n)))
+    out = np.zeros(n.value, dtype=POINT_DTYPE)
+    check(lib().tdlb_rushton_generate(ctypes.byref(p), group, angle, delta_angle,
+                                      out.ctypes.data if n.value else None, n.value, ctypes.byref(n)))
+    return out
+
+
+def rushton_step_increment(p, step):
+    d = ctypes.c_double()
+    check(lib().tdlb_rushton_step_increment(ctypes.byref(p), step, ctypes.byref(d)))
+    return d.value
+
+
+def rushton_solid_mask(p, xg, yg, zg, ghost=1):
+    out = np.empty(xg * yg * zg, dtype=np.uint8)
+    check(lib().tdlb_rushton_solid_mask(ctypes.byref(p), xg, yg, zg, ghost, out.ctypes.data))
+    return out
+
+
+class ComputeUnit:
+    """One ComputeUnit in HBM: the reference's ComputeUnitBase<T,18,Layout> surface."""
+
+    def __init__(self, x, y=None, z=None, ghost=1, layout=LAYOUT_LIJK, streaming=STREAM_ESOTWIST,
+                 precision=PREC_F32, device_id=0, **flow):
+        p = CuParams()
+        p.x, p.y, p.z = x, (x if y is None else y), (x if z is None else z)
+        p.ghost, p.layout, p.streaming, p.precision, p.device_id = ghost, layout, streaming, precision, device_id
+        p.resolution = 1
+        self.params = p
+        self.flow = make_flow(**flow)
+        self._h = ctypes.c_void_p()
+        check(lib().tdlb_cu_create(ctypes.byref(p), ctypes.byref(self.flow), ctypes.byref(self._h)))
+        xg, yg, zg, size = (ctypes.c_int64() for _ in range(4))
+        check(lib().tdlb_cu_dims(self._h, ctypes.byref(xg), ctypes.byref(yg), ctypes.byref(zg), ctypes.byref(size)))
+        self.xg, self.yg, self.zg, self.size = xg.value, yg.value, zg.value, size.value
+        self.layout = layout
+        self.np_dtype = np.float32 if precision == PREC_F32 else np.float64
+        self.step = 0
+        self.angle = 0.0
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            check(lib().tdlb_cu_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- reference operator surface
+    def set_flow(self, **kw):
+        vals = {n: getattr(self.flow, n) for n, _ in Flow._fields_}
+        vals.update(kw)
+        self.flow = make_flow(**vals)
+        check(lib().tdlb_cu_set_flow(self._h, ctypes.byref(self.flow)))
+
+    def initialise(self, rho):
+        check(lib().tdlb_cu_initialise(self._h, rho))
+
+    def collision(self):
+        check(lib().tdlb_cu_collision(self._h))
+
+    def moments(self):
+        check(lib().tdlb_cu_moments(self._h))
+
+    def streaming_push(self):
+        check(lib().tdlb_cu_stream_push(self._h))
+
+    def streaming_pull(self):
+        check(lib().tdlb_cu_stream_pull(self._h))
+
+    def bounce_back(self):
+        check(lib().tdlb_cu_bounce_back(self._h))
+
+    def forcing(self, points, alfa, beta):
+        pts = np.ascontiguousarray(points, dtype=POINT_DTYPE)
+        check(lib().tdlb_cu_forcing(self._h, pts.ctypes.data if len(pts) else None, len(pts), alfa, beta))
+
+    @property
+    def even_step(self):
+        e = ctypes.c_int()
+        check(lib().tdlb_cu_get_even_step(self._h, ctypes.byref(e)))
+        return bool(e.value)
+
+    @even_step.setter
+    def even_step(self, v):
+        check(lib().tdlb_cu_set_even_step(self._h, 1 if v else 0))
+
+    def synchronize(self):
+        check(lib().tdlb_cu_synchronize(self._h))
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        check(lib().tdlb_cu_stream(self._h, ctypes.byref(s)))
+        return s.value
+
+    # ---- fields in the reference's in-memory layout
+    def field_bytes(self, field):
+        b = ctypes.c_size_t()
+        check(lib().tdlb_cu_field_bytes(self._h, field, ctypes.byref(b)))
+        return b.value
+
+    def get(self, field, dtype=None):
+        dt = np.uint8 if field == FIELD_O else (dtype or self.np_dtype)
+        nb = self.field_bytes(field)
+        out = np.empty(nb // np.dtype(dt).itemsize, dtype=dt)
+        check(lib().tdlb_cu_download(self._h, field, out.ctypes.data, nb))
+        return out
+
+    def set(self, field, arr):
+        a = np.ascontiguousarray(arr)
+        if field == FIELD_O:
+            a = np.ascontiguousarray(a, dtype=np.uint8)
+        nb = self.field_bytes(field)
+        if a.nbytes != nb:
+            raise TdlbError(f"upload needs {nb} bytes, got {a.nbytes}")
+        check(lib().tdlb_cu_upload(self._h, field, a.ctypes.data, nb))
+
+    def q_ijkl(self, dtype=None):
+        q = self.get(FIELD_Q, dtype)
+        if self.layout == LAYOUT_IJKL:
+            return q.reshape(self.xg, self.yg, self.zg, 18)
+        return np.moveaxis(q.reshape(18, self.xg, self.yg, self.zg), 0, -1)
+
+    def set_q_ijkl(self, q4):
+        q4 = np.asarray(q4)
+        if self.layout == LAYOUT_IJKL:
+            self.set(FIELD_Q, np.ascontiguousarray(q4).reshape(-1))
+        else:
+            self.set(FIELD_Q, np.ascontiguousarray(np.moveaxis(q4, -1, 0)).reshape(-1))
+
+    # ---- fused main loop with the bound Rushton tank
+    def bind_rushton(self, rp, step=1, angle=0.0, alfa=None, beta=None):
+        self.step, self.angle = step, angle
+        check(lib().tdlb_cu_bind_rushton(self._h, ctypes.byref(rp), step, angle,
+                                         self.flow.alpha if alfa is None else alfa,
+                                         self.flow.beta if beta is None else beta))
+
+    def run(self, nsteps):
+        s = ctypes.c_uint64(self.step)
+        a = ctypes.c_double(self.angle)
+        check(lib().tdlb_cu_run(self._h, nsteps, ctypes.byref(s), ctypes.byref(a)))
+        self.step, self.angle = s.value, a.value
+
+    def rotating_points(self):
+        n = ctypes.c_int64()
+        check(lib().tdlb_cu_rotating_points(self._h, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype=POINT_DTYPE)
+        check(lib().tdlb_cu_rotating_points(self._h, out.ctypes.data, n.value, ctypes.byref(n)))
+        return out
+
+    def solid_mask(self):
+        out = np.empty(self.size, dtype=np.uint8)
+        check(lib().tdlb_cu_solid_mask(self._h, out.ctypes.data, out.nbytes))
+        return out
+
+    def profile(self, enable=True):
+        check(lib().tdlb_cu_profile(self._h, 1 if enable else 0))
+
+    def profile_read(self):
+        """(collide_ms, collide_launches, forcing_ms, forcing_passes) since profile(True)."""
+        a, c = ctypes.c_double(), ctypes.c_double()
+        b, d = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().tdlb_cu_profile_read(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                         ctypes.byref(d)))
+        return a.value, b.value, c.value, d.value
+
+    def forcing_info(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib().tdlb_cu_forcing_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value

@@ -1,0 +1,1016 @@
+// C ABI of the MI355X ComputeUnit (declared in include/tdlb.h).
+//
+// State of a handle.  Q[cur] holds the ComputeUnit's Q array in the reference's
+// layout.  Two phases:
+//   explicit  Q[cur] is byte for byte what the reference's Q would hold;
+//   lazy      left by the fused main-loop path: Q[cur] holds post-collision
+//             populations (Simple) or streamed populations (Esotwist) and the
+//             forcing of the step has been applied; the reference's Q is
+//             moments() of the streamed populations.  Any call that reads or
+//             changes Q as the reference sees it first materialises it.
+// Simple uses two lattices: the reference's in-place push equals a pull into
+// the other buffer (StreamingSimple.hpp:76-126).  In the lazy phase the ghost
+// layers of both buffers hold the slots the reference never rewrites from the
+// interior (bounce-back rims, never-written slots); materialising writes the
+// rest of the ghost layer (pushed and bounced-back slots) from the interior.
+//
+// F occupancy: fmask[fm] has a bit per cell where F may be non-zero; F of every
+// other cell is zero in the reference even if the array still holds an old
+// value (forcing's "F = 0 where O == 0", Forcing.hpp:141-151).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/tdlb.h"
+#include "kernels.h"
+#include "rushton.h"
+
+using namespace tdlb;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(TDLB_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+#define CHK(expr)                  \
+    do {                           \
+        int r_ = (expr);           \
+        if (r_ != TDLB_OK) return r_; \
+    } while (0)
+
+enum Phase { PH_EXPLICIT = 0, PH_LAZY = 1 };
+
+struct DevPoints {
+    PointsDev d{};
+    int64_t cap = 0;
+    size_t tsize = 4;
+    void release() {
+        void* ptrs[] = {d.i, d.j, d.k, d.ifrac, d.kfrac, d.u, d.v, d.w};
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+        d = PointsDev{};
+        cap = 0;
+    }
+    int reserve(int64_t n, size_t ts) {
+        if (n <= cap && ts == tsize) return TDLB_OK;
+        release();
+        tsize = ts;
+        const int64_t c = std::max<int64_t>(n, 256);
+        void** ip[] = {(void**)&d.i, (void**)&d.j, (void**)&d.k};
+        for (void** p : ip)
+            if (hipMalloc(p, sizeof(int32_t) * c) != hipSuccess) return fail(TDLB_ERR_NOMEM, "IB point buffers");
+        void** fp[] = {&d.ifrac, &d.kfrac, &d.u, &d.v, &d.w};
+        for (void** p : fp)
+            if (hipMalloc(p, ts * c) != hipSuccess) return fail(TDLB_ERR_NOMEM, "IB point buffers");
+        cap = c;
+        return TDLB_OK;
+    }
+};
+
+// Host-side copy of points in the CU's type, structure of arrays.
+struct HostPoints {
+    std::vector<int32_t> i, j, k;
+    std::vector<uint8_t> ifrac, kfrac, u, v, w;
+    void resize(size_t n, size_t ts) {
+        i.resize(n); j.resize(n); k.resize(n);
+        ifrac.resize(n * ts); kfrac.resize(n * ts); u.resize(n * ts); v.resize(n * ts); w.resize(n * ts);
+    }
+};
+
+template <typename T>
+void put(std::vector<uint8_t>& v, size_t idx, double x) {
+    T t = (T)x;
+    std::memcpy(v.data() + idx * sizeof(T), &t, sizeof(T));
+}
+
+}  // namespace
+
+struct tdlb_cu {
+    tdlb_cu_params prm{};
+    tdlb_flow flow{};
+    Grid g{};
+    int64_t xg = 0, yg = 0, zg = 0;
+    size_t tsize = 4;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    void* Q[2] = {nullptr, nullptr};
+    int cur = 0;
+    void* F = nullptr;
+    void* Nu = nullptr;
+    uint8_t* O = nullptr;
+    uint64_t* fmask[2] = {nullptr, nullptr};
+    uint64_t* gmask = nullptr;
+    uint64_t* static_bits = nullptr;
+    uint64_t* static_base = nullptr;
+    int fm = 0;
+    size_t mask_words = 0;
+
+    bool even = true;
+    Phase phase = PH_EXPLICIT;
+    bool o_dirty = false;         // O may hold non-zero interior values
+    bool static_in_prev = false;  // every static stencil cell is set in fmask[fm]
+
+    // bound Rushton tank
+    bool bound = false;
+    tdlb_rushton_params rp{};
+    DevPoints pts;                       // static points [0, n_static), blades after
+    int64_t n_static = 0, n_rot = 0;
+    std::vector<int64_t> static_levels;  // CSR offsets into [0, n_static)
+    std::vector<int64_t> rot_levels;     // CSR offsets into [n_static, n_static + n_rot)
+    BladeLayout blade{};
+    double alfa = 0.97, beta = 1.9;
+
+    DevPoints api_pts;  // points of the generic forcing call
+
+    // event timing of tdlb_cu_run (tdlb_cu_profile)
+    bool prof = false;
+    std::vector<hipEvent_t> events;        // pool, recorded in pairs
+    std::vector<std::pair<int, int>> coll, force;
+    size_t ev_used = 0;
+
+    size_t q_bytes() const { return (size_t)g.size * kQ * tsize; }
+    size_t f_bytes() const { return (size_t)g.size * 3 * tsize; }
+    size_t nu_bytes() const { return (size_t)g.size * tsize; }
+    size_t mask_bytes() const { return mask_words * sizeof(uint64_t); }
+    bool simple() const { return prm.streaming == TDLB_STREAM_SIMPLE; }
+    int prec() const { return prm.precision; }
+
+    CollideArgs cargs() const {
+        CollideArgs c;
+        c.nu = flow.nu;
+        c.cs0 = flow.cs0;
+        c.g3 = flow.g3;
+        c.use_les = flow.use_les;
+        return c;
+    }
+};
+
+namespace {
+
+int set_device(tdlb_cu* cu) {
+    HIPCHK(hipSetDevice(cu->device));
+    return TDLB_OK;
+}
+
+void release(tdlb_cu* cu) {
+    void* ptrs[] = {cu->Q[0], cu->Q[1], cu->F, cu->Nu, cu->O, cu->fmask[0], cu->fmask[1],
+                    cu->gmask, cu->static_bits, cu->static_base};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    cu->pts.release();
+    cu->api_pts.release();
+    for (hipEvent_t e : cu->events) (void)hipEventDestroy(e);
+    cu->events.clear();
+    if (cu->stream) (void)hipStreamDestroy(cu->stream);
+}
+
+// record a timing event on the handle's stream; returns its pool index or -1
+int prof_mark(tdlb_cu* cu) {
+    if (!cu->prof) return -1;
+    constexpr size_t kMaxEvents = 1 << 16;
+    if (cu->ev_used == cu->events.size()) {
+        if (cu->events.size() >= kMaxEvents) return -1;
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        cu->events.push_back(e);
+    }
+    hipEvent_t e = cu->events[cu->ev_used];
+    if (hipEventRecord(e, cu->stream) != hipSuccess) return -1;
+    return (int)cu->ev_used++;
+}
+
+// ---- phase handling
+int materialise(tdlb_cu* cu) {
+    if (cu->phase != PH_LAZY) return TDLB_OK;
+    const auto c = cu->cargs();
+    if (cu->simple()) {
+        void* src = cu->Q[cu->cur];
+        void* dst = cu->Q[1 - cu->cur];
+        HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_MOM_PULL, src, dst, cu->F, cu->Nu,
+                             cu->fmask[cu->fm], cu->g, c, cu->stream));
+        HIPCHK(launch_ghost_fill((int)cu->tsize, cu->prm.layout, src, dst, cu->g, cu->stream));
+        cu->cur = 1 - cu->cur;
+    } else {
+        HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_MOM, cu->even ? 1 : 0, cu->Q[0], cu->F, cu->Nu,
+                               cu->fmask[cu->fm], cu->g, c, cu->stream));
+    }
+    cu->phase = PH_EXPLICIT;
+    return TDLB_OK;
+}
+
+// ---- IB levels: a point's level is one more than the highest level of any
+// earlier point whose 3x3 (i,k) stencil shares a cell with it, so the points of
+// a level are independent and levels run in order (Forcing.hpp:73-135 is a
+// sequential loop over the list).
+void stencil_cells(const tdlb_cu* cu, int64_t i, int64_t j, int64_t k, uint64_t* cells) {
+    int t = 0;
+    for (int k1 = -1; k1 <= 1; k1++)
+        for (int i1 = -1; i1 <= 1; i1++) {
+            int64_t i2 = i + i1, k2 = k + k1;
+            if (i2 == 0) i2 = cu->g.xg1;
+            if (i2 == cu->g.xg1 + 1) i2 = 1;
+            if (k2 == 0) k2 = cu->g.zg1;
+            if (k2 == cu->g.zg1 + 1) k2 = 1;
+            cells[t++] = ((uint64_t)i2 * cu->yg + (uint64_t)j) * cu->zg + (uint64_t)k2;
+        }
+}
+
+int check_points(const tdlb_cu* cu, const tdlb_ib_point* p, int64_t n) {
+    const int64_t gh = cu->prm.ghost;
+    for (int64_t q = 0; q < n; q++) {
+        const int64_t i = p[q].i + gh, j = p[q].j + gh, k = p[q].k + gh;
+        if (i < 1 || i > cu->g.xg1 || j < 1 || j > cu->g.yg1 || k < 1 || k > cu->g.zg1)
+            return fail(TDLB_ERR_INVALID, "IB point " + std::to_string(q) + " outside the interior: (" +
+                                              std::to_string(p[q].i) + "," + std::to_string(p[q].j) + "," +
+                                              std::to_string(p[q].k) + ")");
+    }
+    return TDLB_OK;
+}
+
+// levels of a list; returns the permutation ordering points by level (stable)
+// and CSR offsets (offsets[0] = 0 .. offsets[L] = n)
+void compute_levels(const tdlb_cu* cu, const tdlb_ib_point* p, int64_t n, std::vector<int64_t>& order,
+                    std::vector<int64_t>& offsets, int base_level = 0) {
+    std::unordered_map<uint64_t, int32_t> last;
+    last.reserve((size_t)n * 4 + 16);
+    std::vector<int32_t> lev(n);
+    int32_t maxl = -1;
+    uint64_t cells[9];
+    const int64_t gh = cu->prm.ghost;
+    for (int64_t q = 0; q < n; q++) {
+        stencil_cells(cu, p[q].i + gh, p[q].j + gh, p[q].k + gh, cells);
+        int32_t l = base_level;
+        for (uint64_t c : cells) {
+            auto it = last.find(c);
+            if (it != last.end()) l = std::max(l, it->second + 1);
+        }
+        for (uint64_t c : cells) last[c] = l;
+        lev[q] = l;
+        maxl = std::max(maxl, l);
+    }
+    const int nl = maxl + 1 - base_level;
+    offsets.assign(std::max(nl, 0) + 1, 0);
+    for (int64_t q = 0; q < n; q++) offsets[lev[q] - base_level + 1]++;
+    for (int l = 0; l < nl; l++) offsets[l + 1] += offsets[l];
+    order.resize(n);
+    std::vector<int64_t> fill(offsets.begin(), offsets.end() - 1);
+    for (int64_t q = 0; q < n; q++) order[fill[lev[q] - base_level]++] = q;
+}
+
+void to_host_points(const tdlb_cu* cu, const tdlb_ib_point* p, const std::vector<int64_t>& order, HostPoints& h) {
+    const size_t n = order.size();
+    h.resize(n, cu->tsize);
+    const int64_t gh = cu->prm.ghost;
+    for (size_t a = 0; a < n; a++) {
+        const tdlb_ib_point& q = p[order[a]];
+        h.i[a] = (int32_t)(q.i + gh);
+        h.j[a] = (int32_t)(q.j + gh);
+        h.k[a] = (int32_t)(q.k + gh);
+        if (cu->tsize == 4) {
+            put<float>(h.ifrac, a, q.i_frac); put<float>(h.kfrac, a, q.k_frac);
+            put<float>(h.u, a, q.u_delta); put<float>(h.v, a, q.v_delta); put<float>(h.w, a, q.w_delta);
+        } else {
+            put<double>(h.ifrac, a, q.i_frac); put<double>(h.kfrac, a, q.k_frac);
+            put<double>(h.u, a, q.u_delta); put<double>(h.v, a, q.v_delta); put<double>(h.w, a, q.w_delta);
+        }
+    }
+}
+
+int upload_points(tdlb_cu* cu, DevPoints& dp, int64_t offset, const HostPoints& h) {
+    const size_t n = h.i.size();
+    if (n == 0) return TDLB_OK;
+    const size_t ts = cu->tsize;
+    HIPCHK(hipMemcpyAsync(dp.d.i + offset, h.i.data(), n * 4, hipMemcpyHostToDevice, cu->stream));
+    HIPCHK(hipMemcpyAsync(dp.d.j + offset, h.j.data(), n * 4, hipMemcpyHostToDevice, cu->stream));
+    HIPCHK(hipMemcpyAsync(dp.d.k + offset, h.k.data(), n * 4, hipMemcpyHostToDevice, cu->stream));
+    HIPCHK(hipMemcpyAsync((char*)dp.d.ifrac + offset * ts, h.ifrac.data(), n * ts, hipMemcpyHostToDevice, cu->stream));
+    HIPCHK(hipMemcpyAsync((char*)dp.d.kfrac + offset * ts, h.kfrac.data(), n * ts, hipMemcpyHostToDevice, cu->stream));
+    HIPCHK(hipMemcpyAsync((char*)dp.d.u + offset * ts, h.u.data(), n * ts, hipMemcpyHostToDevice, cu->stream));
+    HIPCHK(hipMemcpyAsync((char*)dp.d.v + offset * ts, h.v.data(), n * ts, hipMemcpyHostToDevice, cu->stream));
+    HIPCHK(hipMemcpyAsync((char*)dp.d.w + offset * ts, h.w.data(), n * ts, hipMemcpyHostToDevice, cu->stream));
+    HIPCHK(hipStreamSynchronize(cu->stream));  // host staging buffers end with the caller
+    return TDLB_OK;
+}
+
+// One forcing pass.  Levels: list of (begin, end) point ranges run in order.
+// prepare: point ranges whose stencil cells may be missing from the previous
+// occupancy mask.  base: bits the new mask starts from.
+int forcing_pass(tdlb_cu* cu, const DevPoints& dp, const std::vector<std::pair<int64_t, int64_t>>& levels,
+                 const std::vector<std::pair<int64_t, int64_t>>& prepare, const uint64_t* base, int momsrc) {
+    uint64_t* prev = cu->fmask[cu->fm];
+    uint64_t* next = cu->fmask[1 - cu->fm];
+    HIPCHK(launch_copy(next, base, cu->mask_bytes(), cu->stream));
+    for (auto& r : prepare)
+        HIPCHK(launch_force_prepare(cu->prec(), dp.d, r.first, r.second - r.first, cu->F, prev, next, cu->g,
+                                    cu->stream));
+    const void* Qsrc = cu->Q[cu->cur];
+    for (auto& r : levels)
+        HIPCHK(launch_force_level(cu->prec(), cu->prm.layout, cu->prm.streaming, momsrc, cu->even ? 1 : 0, dp.d,
+                                  r.first, r.second - r.first, cu->F, Qsrc, cu->g, cu->alfa, cu->beta,
+                                  cu->stream));
+    if (cu->o_dirty) {
+        HIPCHK(launch_zero_interior_o(cu->O, cu->g, cu->stream));
+        cu->o_dirty = false;
+    }
+    cu->fm = 1 - cu->fm;
+    return TDLB_OK;
+}
+
+int bound_forcing(tdlb_cu* cu, int momsrc, bool prepare_static) {
+    std::vector<std::pair<int64_t, int64_t>> levels, prep;
+    for (size_t l = 0; l + 1 < cu->static_levels.size(); l++)
+        levels.push_back({cu->static_levels[l], cu->static_levels[l + 1]});
+    for (size_t l = 0; l + 1 < cu->rot_levels.size(); l++)
+        levels.push_back({cu->rot_levels[l], cu->rot_levels[l + 1]});
+    if (prepare_static) prep.push_back({0, cu->n_static});
+    prep.push_back({cu->n_static, cu->n_static + cu->n_rot});
+    CHK(forcing_pass(cu, cu->pts, levels, prep, cu->static_base, momsrc));
+    cu->static_in_prev = true;
+    return TDLB_OK;
+}
+
+int gen_blades(tdlb_cu* cu, double angle, double omega) {
+    BladeStep st{};
+    Rushton R(cu->rp);
+    R.blade_trig(angle, st.cosb, st.sinb);
+    st.omega = omega;
+    HIPCHK(launch_gen_blades(cu->prec(), cu->blade, st, cu->pts.d, cu->n_static, cu->stream));
+    return TDLB_OK;
+}
+
+// reference precision of an angular increment (main.cpp uses useQVecPrecision)
+double to_prec(const tdlb_cu* cu, double v) { return cu->tsize == 4 ? (double)(float)v : v; }
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* tdlb_last_error(void) { return g_err.c_str(); }
+int tdlb_version(void) { return 1; }
+
+int tdlb_device_count(int* count) {
+    if (!count) return fail(TDLB_ERR_INVALID, "null count");
+    HIPCHK(hipGetDeviceCount(count));
+    return TDLB_OK;
+}
+
+int tdlb_flow_calc_nu_re_m(tdlb_flow* f, int32_t outer_radius, int32_t precision) {
+    if (!f) return fail(TDLB_ERR_INVALID, "null flow");
+    if (precision == TDLB_PREC_F32) {
+        const float reM = (float)f->re_m_nondimensional, uav = (float)f->uav;
+        const float Re_m = reM * M_PI / 2.0;
+        const float nu = uav * static_cast<float>(outer_radius) / Re_m;
+        f->re_m = Re_m;
+        f->nu = nu;
+    } else if (precision == TDLB_PREC_F64) {
+        const double Re_m = f->re_m_nondimensional * M_PI / 2.0;
+        f->re_m = Re_m;
+        f->nu = f->uav * static_cast<double>(outer_radius) / Re_m;
+    } else {
+        return fail(TDLB_ERR_INVALID, "precision");
+    }
+    return TDLB_OK;
+}
+
+int tdlb_cu_create(const tdlb_cu_params* p, const tdlb_flow* f, tdlb_cu_t* out) {
+    if (!p || !f || !out) return fail(TDLB_ERR_INVALID, "null argument");
+    if (p->ghost != 1) return fail(TDLB_ERR_UNSUPPORTED, "ghost must be 1 (GridParams::multiStep default)");
+    if (p->x < 1 || p->y < 1 || p->z < 1) return fail(TDLB_ERR_INVALID, "grid must have at least one interior cell");
+    if (p->layout != TDLB_LAYOUT_IJKL && p->layout != TDLB_LAYOUT_LIJK) return fail(TDLB_ERR_INVALID, "layout");
+    if (p->streaming != TDLB_STREAM_SIMPLE && p->streaming != TDLB_STREAM_ESOTWIST)
+        return fail(TDLB_ERR_INVALID, "streaming");
+    if (p->precision != TDLB_PREC_F32 && p->precision != TDLB_PREC_F64)
+        return fail(TDLB_ERR_UNSUPPORTED, "precision");
+    if (f->collision != 0) return fail(TDLB_ERR_UNSUPPORTED, "only the EgglesSomers collision is implemented");
+    const int64_t xg = p->x + 2, yg = p->y + 2, zg = p->z + 2;
+    const uint64_t size = (uint64_t)xg * yg * zg;
+    if (size >= (1ull << 32) / kQ) return fail(TDLB_ERR_UNSUPPORTED, "grid too large for 32-bit cell indexing");
+
+    tdlb_cu* cu = new tdlb_cu();
+    cu->prm = *p;
+    cu->flow = *f;
+    cu->xg = xg; cu->yg = yg; cu->zg = zg;
+    cu->g.xg = (int32_t)xg; cu->g.yg = (int32_t)yg; cu->g.zg = (int32_t)zg;
+    cu->g.xg1 = (int32_t)(xg - 2); cu->g.yg1 = (int32_t)(yg - 2); cu->g.zg1 = (int32_t)(zg - 2);
+    cu->g.yz = (uint32_t)(yg * zg);
+    cu->g.mask_w = (uint32_t)((zg + 63) / 64);
+    cu->g.size = size;
+    cu->tsize = p->precision == TDLB_PREC_F32 ? 4 : 8;
+    cu->mask_words = (size_t)xg * yg * cu->g.mask_w;
+    cu->device = p->device_id < 0 ? 0 : p->device_id;
+    cu->alfa = f->alpha;
+    cu->beta = f->beta;
+
+    auto bad = [&](int code, const std::string& m) {
+        release(cu);
+        delete cu;
+        return fail(code, m);
+    };
+    if (hipSetDevice(cu->device) != hipSuccess) return bad(TDLB_ERR_HIP, "hipSetDevice failed");
+    if (hipStreamCreateWithFlags(&cu->stream, hipStreamNonBlocking) != hipSuccess)
+        return bad(TDLB_ERR_HIP, "hipStreamCreate failed");
+    const int nq = cu->simple() ? 2 : 1;
+    for (int b = 0; b < nq; b++)
+        if (hipMalloc(&cu->Q[b], cu->q_bytes()) != hipSuccess) return bad(TDLB_ERR_NOMEM, "hipMalloc Q");
+    if (hipMalloc(&cu->F, cu->f_bytes()) != hipSuccess) return bad(TDLB_ERR_NOMEM, "hipMalloc F");
+    if (hipMalloc(&cu->Nu, cu->nu_bytes()) != hipSuccess) return bad(TDLB_ERR_NOMEM, "hipMalloc Nu");
+    if (hipMalloc((void**)&cu->O, size) != hipSuccess) return bad(TDLB_ERR_NOMEM, "hipMalloc O");
+    uint64_t** masks[] = {&cu->fmask[0], &cu->fmask[1], &cu->gmask};
+    for (uint64_t** m : masks)
+        if (hipMalloc((void**)m, cu->mask_bytes()) != hipSuccess) return bad(TDLB_ERR_NOMEM, "hipMalloc mask");
+    // a fresh ComputeUnit holds zero F/Nu/O (its arrays are unset in the reference; zero is a valid state)
+    if (launch_zero(cu->F, cu->f_bytes(), cu->stream) != hipSuccess ||
+        launch_zero(cu->Nu, cu->nu_bytes(), cu->stream) != hipSuccess ||
+        launch_zero(cu->O, size, cu->stream) != hipSuccess ||
+        launch_zero(cu->fmask[0], cu->mask_bytes(), cu->stream) != hipSuccess ||
+        launch_zero(cu->fmask[1], cu->mask_bytes(), cu->stream) != hipSuccess ||
+        launch_zero(cu->gmask, cu->mask_bytes(), cu->stream) != hipSuccess)
+        return bad(TDLB_ERR_HIP, "initial fill failed");
+    for (int b = 0; b < nq; b++)
+        if (launch_zero(cu->Q[b], cu->q_bytes(), cu->stream) != hipSuccess) return bad(TDLB_ERR_HIP, "Q fill failed");
+    if (hipStreamSynchronize(cu->stream) != hipSuccess) return bad(TDLB_ERR_HIP, "synchronise failed");
+    *out = cu;
+    return TDLB_OK;
+}
+
+int tdlb_cu_destroy(tdlb_cu_t cu) {
+    if (!cu) return TDLB_OK;
+    (void)hipSetDevice(cu->device);
+    if (cu->stream) (void)hipStreamSynchronize(cu->stream);
+    release(cu);
+    delete cu;
+    return TDLB_OK;
+}
+
+int tdlb_cu_set_flow(tdlb_cu_t cu, const tdlb_flow* f) {
+    if (!cu || !f) return fail(TDLB_ERR_INVALID, "null argument");
+    if (f->collision != 0) return fail(TDLB_ERR_UNSUPPORTED, "only the EgglesSomers collision is implemented");
+    CHK(set_device(cu));
+    CHK(materialise(cu));
+    cu->flow = *f;
+    return TDLB_OK;
+}
+
+int tdlb_cu_initialise(tdlb_cu_t cu, double rho) {
+    if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    CHK(set_device(cu));
+    const double r = to_prec(cu, rho);
+    const int nq = cu->simple() ? 2 : 1;
+    for (int b = 0; b < nq; b++) HIPCHK(launch_fill_q(cu->prec(), cu->prm.layout, cu->Q[b], cu->g, r, cu->stream));
+    HIPCHK(launch_zero(cu->F, cu->f_bytes(), cu->stream));
+    HIPCHK(launch_zero(cu->Nu, cu->nu_bytes(), cu->stream));
+    HIPCHK(launch_zero(cu->O, cu->g.size, cu->stream));
+    HIPCHK(launch_zero(cu->fmask[0], cu->mask_bytes(), cu->stream));
+    HIPCHK(launch_zero(cu->fmask[1], cu->mask_bytes(), cu->stream));
+    HIPCHK(launch_zero(cu->gmask, cu->mask_bytes(), cu->stream));
+    cu->phase = PH_EXPLICIT;
+    cu->o_dirty = false;
+    cu->static_in_prev = false;
+    if (cu->bound) HIPCHK(launch_copy(cu->static_base, cu->static_bits, cu->mask_bytes(), cu->stream));
+    return TDLB_OK;
+}
+
+int tdlb_cu_collision(tdlb_cu_t cu) {
+    if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    CHK(set_device(cu));
+    CHK(materialise(cu));
+    const auto c = cu->cargs();
+    if (cu->simple())
+        HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_COLL, cu->Q[cu->cur], cu->Q[cu->cur], cu->F, cu->Nu,
+                             cu->fmask[cu->fm], cu->g, c, cu->stream));
+    else
+        HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_COLL, cu->even ? 1 : 0, cu->Q[0], cu->F, cu->Nu,
+                               cu->fmask[cu->fm], cu->g, c, cu->stream));
+    return TDLB_OK;
+}
+
+int tdlb_cu_moments(tdlb_cu_t cu) {
+    if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    CHK(set_device(cu));
+    CHK(materialise(cu));
+    const auto c = cu->cargs();
+    if (cu->simple())
+        HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_MOM, cu->Q[cu->cur], cu->Q[cu->cur], cu->F, cu->Nu,
+                             cu->fmask[cu->fm], cu->g, c, cu->stream));
+    else
+        HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_MOM, cu->even ? 1 : 0, cu->Q[0], cu->F, cu->Nu,
+                               cu->fmask[cu->fm], cu->g, c, cu->stream));
+    return TDLB_OK;
+}
+
+int tdlb_cu_stream_push(tdlb_cu_t cu) {
+    if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    CHK(set_device(cu));
+    CHK(materialise(cu));
+    if (!cu->simple()) {
+        cu->even = !cu->even;
+        return TDLB_OK;
+    }
+    HIPCHK(launch_push_all((int)cu->tsize, cu->prm.layout, cu->Q[cu->cur], cu->Q[1 - cu->cur], cu->g, cu->stream));
+    cu->cur = 1 - cu->cur;
+    return TDLB_OK;
+}
+
+int tdlb_cu_stream_pull(tdlb_cu_t cu) {
+    if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    CHK(set_device(cu));
+    CHK(materialise(cu));
+    if (!cu->simple()) {
+        cu->even = !cu->even;
+        return TDLB_OK;
+    }
+    HIPCHK(launch_pull_all((int)cu->tsize, cu->prm.layout, cu->Q[cu->cur], cu->Q[1 - cu->cur], cu->g, cu->stream));
+    cu->cur = 1 - cu->cur;
+    return TDLB_OK;
+}
+
+int tdlb_cu_bounce_back(tdlb_cu_t cu) {
+    if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    CHK(set_device(cu));
+    CHK(materialise(cu));
+    if (!cu->simple()) return TDLB_OK;  // Boundary.hpp:13-14
+    HIPCHK(launch_bounce_back((int)cu->tsize, cu->prm.layout, cu->Q[cu->cur], cu->g, cu->stream));
+    return TDLB_OK;
+}
+
+int tdlb_cu_forcing(tdlb_cu_t cu, const tdlb_ib_point* points, int64_t n, double alfa, double beta) {
+    if (!cu || (n > 0 && !points) || n < 0) return fail(TDLB_ERR_INVALID, "bad arguments");
+    CHK(set_device(cu));
+    CHK(check_points(cu, points, n));
+    CHK(materialise(cu));
+    std::vector<int64_t> order, offsets;
+    compute_levels(cu, points, n, order, offsets);
+    HostPoints h;
+    to_host_points(cu, points, order, h);
+    CHK(cu->api_pts.reserve(n, cu->tsize));
+    CHK(upload_points(cu, cu->api_pts, 0, h));
+    std::vector<std::pair<int64_t, int64_t>> levels, prep;
+    for (size_t l = 0; l + 1 < offsets.size(); l++) levels.push_back({offsets[l], offsets[l + 1]});
+    if (n > 0) prep.push_back({0, n});
+    const double a0 = cu->alfa, b0 = cu->beta;
+    cu->alfa = alfa;
+    cu->beta = beta;
+    int r = forcing_pass(cu, cu->api_pts, levels, prep, cu->gmask, MOMSRC_STORED);
+    cu->alfa = a0;
+    cu->beta = b0;
+    cu->static_in_prev = false;
+    return r;
+}
+
+int tdlb_cu_field_bytes(tdlb_cu_t cu, int field, size_t* bytes) {
+    if (!cu || !bytes) return fail(TDLB_ERR_INVALID, "null argument");
+    switch (field) {
+        case TDLB_FIELD_Q: *bytes = cu->q_bytes(); break;
+        case TDLB_FIELD_F: *bytes = cu->f_bytes(); break;
+        case TDLB_FIELD_NU: *bytes = cu->nu_bytes(); break;
+        case TDLB_FIELD_O: *bytes = cu->g.size; break;
+        default: return fail(TDLB_ERR_INVALID, "field");
+    }
+    return TDLB_OK;
+}
+
+int tdlb_cu_download(tdlb_cu_t cu, int field, void* host, size_t bytes) {
+    size_t need = 0;
+    CHK(tdlb_cu_field_bytes(cu, field, &need));
+    if (!host || bytes != need) return fail(TDLB_ERR_INVALID, "download: buffer size must be " + std::to_string(need));
+    CHK(set_device(cu));
+    CHK(materialise(cu));
+    if (field == TDLB_FIELD_F) {
+        void* tmp = nullptr;
+        if (hipMalloc(&tmp, need) != hipSuccess) return fail(TDLB_ERR_NOMEM, "download F staging");
+        hipError_t e = launch_masked_f(cu->prec(), cu->F, cu->fmask[cu->fm], tmp, cu->g, cu->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(host, tmp, need, hipMemcpyDeviceToHost, cu->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(cu->stream);
+        (void)hipFree(tmp);
+        if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("download F: ") + hipGetErrorString(e));
+        return TDLB_OK;
+    }
+    const void* src = field == TDLB_FIELD_Q ? cu->Q[cu->cur] : field == TDLB_FIELD_NU ? cu->Nu : (const void*)cu->O;
+    HIPCHK(hipMemcpyAsync(host, src, need, hipMemcpyDeviceToHost, cu->stream));
+    HIPCHK(hipStreamSynchronize(cu->stream));
+    return TDLB_OK;
+}
+
+int tdlb_cu_upload(tdlb_cu_t cu, int field, const void* host, size_t bytes) {
+    size_t need = 0;
+    CHK(tdlb_cu_field_bytes(cu, field, &need));
+    if (!host || bytes != need) return fail(TDLB_ERR_INVALID, "upload: buffer size must be " + std::to_string(need));
+    CHK(set_device(cu));
+    CHK(materialise(cu));
+    void* dst = field == TDLB_FIELD_Q ? cu->Q[cu->cur] : field == TDLB_FIELD_F ? cu->F
+              : field == TDLB_FIELD_NU ? cu->Nu : (void*)cu->O;
+    HIPCHK(hipMemcpyAsync(dst, host, need, hipMemcpyHostToDevice, cu->stream));
+    if (field == TDLB_FIELD_F) {
+        HIPCHK(launch_mask_from_f(cu->prec(), cu->F, cu->fmask[cu->fm], cu->gmask, cu->g, cu->stream));
+        cu->static_in_prev = false;
+        if (cu->bound) {
+            // static_base = static stencil bits | ghost bits with non-zero F
+            std::vector<uint64_t> a(cu->mask_words), b(cu->mask_words);
+            HIPCHK(hipMemcpyAsync(a.data(), cu->static_bits, cu->mask_bytes(), hipMemcpyDeviceToHost, cu->stream));
+            HIPCHK(hipMemcpyAsync(b.data(), cu->gmask, cu->mask_bytes(), hipMemcpyDeviceToHost, cu->stream));
+            HIPCHK(hipStreamSynchronize(cu->stream));
+            for (size_t w = 0; w < a.size(); w++) a[w] |= b[w];
+            HIPCHK(hipMemcpyAsync(cu->static_base, a.data(), cu->mask_bytes(), hipMemcpyHostToDevice, cu->stream));
+        }
+    }
+    if (field == TDLB_FIELD_O) cu->o_dirty = true;
+    HIPCHK(hipStreamSynchronize(cu->stream));
+    return TDLB_OK;
+}
+
+int tdlb_cu_get_even_step(tdlb_cu_t cu, int* even) {
+    if (!cu || !even) return fail(TDLB_ERR_INVALID, "null argument");
+    CHK(set_device(cu));
+    CHK(materialise(cu));
+    *even = cu->even ? 1 : 0;
+    return TDLB_OK;
+}
+
+int tdlb_cu_set_even_step(tdlb_cu_t cu, int even) {
+    if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    CHK(set_device(cu));
+    CHK(materialise(cu));
+    cu->even = even != 0;
+    return TDLB_OK;
+}
+
+int tdlb_cu_dims(tdlb_cu_t cu, int64_t* xg, int64_t* yg, int64_t* zg, int64_t* size) {
+    if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    if (xg) *xg = cu->xg;
+    if (yg) *yg = cu->yg;
+    if (zg) *zg = cu->zg;
+    if (size) *size = (int64_t)cu->g.size;
+    return TDLB_OK;
+}
+
+int tdlb_cu_synchronize(tdlb_cu_t cu) {
+    if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    CHK(set_device(cu));
+    HIPCHK(hipStreamSynchronize(cu->stream));
+    return TDLB_OK;
+}
+
+int tdlb_cu_stream(tdlb_cu_t cu, void** s) {
+    if (!cu || !s) return fail(TDLB_ERR_INVALID, "null argument");
+    *s = (void*)cu->stream;
+    return TDLB_OK;
+}
+
+// ---------------------------------------------------------------- checkpoint
+static std::string ckpt_id(const tdlb_cu* cu) {
+    return std::to_string(cu->prm.idi) + "." + std::to_string(cu->prm.idj) + "." + std::to_string(cu->prm.idk);
+}
+
+int tdlb_cu_checkpoint_write(tdlb_cu_t cu, const char* dir, const char* unit_name, const char* all_params_json) {
+    if (!cu || !dir || !unit_name) return fail(TDLB_ERR_INVALID, "null argument");
+    const std::string id = ckpt_id(cu);
+    if (all_params_json) {
+        const std::string path = std::string(dir) + "/AllParams." + id + "." + unit_name + ".json";
+        FILE* fp = std::fopen(path.c_str(), "w");
+        if (!fp) return fail(TDLB_ERR_IO, "cannot write " + path);
+        const size_t n = std::strlen(all_params_json);
+        const bool ok = std::fwrite(all_params_json, 1, n, fp) == n;
+        std::fclose(fp);
+        if (!ok) return fail(TDLB_ERR_IO, "short write " + path);
+    }
+    const int fields[3] = {TDLB_FIELD_Q, TDLB_FIELD_F, TDLB_FIELD_NU};
+    const char* names[3] = {"Q", "F", "Nu"};
+    for (int a = 0; a < 3; a++) {
+        size_t bytes = 0;
+        CHK(tdlb_cu_field_bytes(cu, fields[a], &bytes));
+        std::vector<char> buf(bytes);
+        CHK(tdlb_cu_download(cu, fields[a], buf.data(), bytes));
+        const std::string path = std::string(dir) + "/checkpoint_grid." + id + "." + unit_name + "." + names[a];
+        FILE* fp = std::fopen(path.c_str(), "wb");
+        if (!fp) return fail(TDLB_ERR_IO, "cannot write " + path);
+        const bool ok = std::fwrite(buf.data(), 1, bytes, fp) == bytes;
+        std::fclose(fp);
+        if (!ok) return fail(TDLB_ERR_IO, "short write " + path);
+    }
+    return TDLB_OK;
+}
+
+int tdlb_cu_checkpoint_read(tdlb_cu_t cu, const char* dir, const char* unit_name, int* missing) {
+    if (!cu || !dir || !unit_name) return fail(TDLB_ERR_INVALID, "null argument");
+    const std::string id = ckpt_id(cu);
+    const int fields[3] = {TDLB_FIELD_Q, TDLB_FIELD_F, TDLB_FIELD_NU};
+    const char* names[3] = {"Q", "F", "Nu"};
+    int miss = 0;
+    for (int a = 0; a < 3; a++) {
+        size_t bytes = 0;
+        CHK(tdlb_cu_field_bytes(cu, fields[a], &bytes));
+        const std::string path = std::string(dir) + "/checkpoint_grid." + id + "." + unit_name + "." + names[a];
+        FILE* fp = std::fopen(path.c_str(), "rb");
+        if (!fp) {
+            miss |= 1 << a;
+            continue;
+        }
+        std::vector<char> buf(bytes);
+        const size_t got = std::fread(buf.data(), 1, bytes, fp);
+        const bool extra = std::fgetc(fp) != EOF;
+        std::fclose(fp);
+        if (got != bytes || extra)
+            return fail(TDLB_ERR_IO, path + ": expected " + std::to_string(bytes) + " bytes");
+        CHK(tdlb_cu_upload(cu, fields[a], buf.data(), bytes));
+    }
+    if (missing) *missing = miss;
+    return TDLB_OK;
+}
+
+// ---------------------------------------------------------------- geometry
+int tdlb_rushton_defaults(int64_t tank_diameter, int64_t height, double tip_speed, int64_t startup_steps,
+                          tdlb_rushton_params* out) {
+    if (!out || tank_diameter < 6 || height < 1) return fail(TDLB_ERR_INVALID, "tank too small");
+    rushton_defaults(tank_diameter, height, out);
+    out->tip_speed = tip_speed;
+    out->startup_steps = startup_steps;
+    return TDLB_OK;
+}
+
+int tdlb_rushton_generate(const tdlb_rushton_params* p, int group, double angle, double delta_angle,
+                          tdlb_ib_point* out, int64_t cap, int64_t* n) {
+    if (!p || !n) return fail(TDLB_ERR_INVALID, "null argument");
+    Rushton R(*p);
+    std::vector<tdlb_ib_point> v;
+    if (group == TDLB_GEOM_FIXED) R.fixed(v);
+    else if (group == TDLB_GEOM_ROTATING_NON_UPDATING) R.rotating_non_updating(delta_angle, v);
+    else if (group == TDLB_GEOM_ROTATING) R.rotating(angle, delta_angle, v);
+    else return fail(TDLB_ERR_INVALID, "group");
+    *n = (int64_t)v.size();
+    if (out && cap > 0) std::memcpy(out, v.data(), sizeof(tdlb_ib_point) * std::min<int64_t>(cap, *n));
+    return TDLB_OK;
+}
+
+int tdlb_rushton_step_increment(const tdlb_rushton_params* p, uint64_t step, double* delta) {
+    if (!p || !delta) return fail(TDLB_ERR_INVALID, "null argument");
+    *delta = Rushton(*p).step_increment(step);
+    return TDLB_OK;
+}
+
+int tdlb_rushton_solid_mask(const tdlb_rushton_params* p, int64_t xg, int64_t yg, int64_t zg, int64_t ghost,
+                            uint8_t* out) {
+    if (!p || !out) return fail(TDLB_ERR_INVALID, "null argument");
+    Rushton(*p).solid_mask(xg, yg, zg, ghost, out);
+    return TDLB_OK;
+}
+
+// ---------------------------------------------------------------- fused main loop
+int tdlb_cu_bind_rushton(tdlb_cu_t cu, const tdlb_rushton_params* p, uint64_t step, double angle, double alfa,
+                         double beta) {
+    if (!cu || !p) return fail(TDLB_ERR_INVALID, "null argument");
+    CHK(set_device(cu));
+    CHK(materialise(cu));
+    Rushton R(*p);
+    const BladeEnum be = R.blade_enum();
+    if (p->n_blades < 1 || p->n_blades > 16) return fail(TDLB_ERR_INVALID, "n_blades must be 1..16");
+    // Blade colours are pairwise non-overlapping when blades of one colour are
+    // more than 3*sqrt(2) apart at the inner radius.
+    const double sep = 2.0 * p->blade_inner_radius * std::sin(3.14159265358979323846 * be.n_bcol / p->n_blades);
+    if (be.n_bcol * 1 > p->n_blades || sep < 3.0 * std::sqrt(2.0) + 1e-9)
+        return fail(TDLB_ERR_UNSUPPORTED, "tank too small for on-device blade colouring (use tdlb_cu_forcing)");
+    // blades are generated on the device at any angle: their whole swept disk and
+    // the planes they occupy must lie inside the interior
+    {
+        const double ro = p->blade_outer_radius;
+        const double lo_i = std::floor(p->center_x - ro + 0.5), hi_i = std::floor(p->center_x + ro + 0.5);
+        const double lo_k = std::floor(p->center_z - ro + 0.5), hi_k = std::floor(p->center_z + ro + 0.5);
+        if (lo_i < 0 || hi_i > (double)(cu->prm.x - 1) || lo_k < 0 || hi_k > (double)(cu->prm.z - 1) ||
+            p->blade_plane_lo < 0 || p->blade_plane_lo + p->blade_planes > cu->prm.y || p->blade_planes < 1 ||
+            p->blade_inner_radius < 0)
+            return fail(TDLB_ERR_INVALID, "impeller does not fit inside the ComputeUnit interior");
+    }
+
+    // main.cpp:277-303: increment for the current step, fixed, non-updating, rotating
+    const double delta = to_prec(cu, R.step_increment(step));
+    std::vector<tdlb_ib_point> stat, rot;
+    R.fixed(stat);
+    R.rotating_non_updating(delta, stat);
+    R.rotating(angle, delta, rot);
+    CHK(check_points(cu, stat.data(), (int64_t)stat.size()));
+    CHK(check_points(cu, rot.data(), (int64_t)rot.size()));
+
+    std::vector<int64_t> order, offs;
+    compute_levels(cu, stat.data(), (int64_t)stat.size(), order, offs);
+    HostPoints hs;
+    to_host_points(cu, stat.data(), order, hs);
+    cu->n_static = (int64_t)stat.size();
+    cu->n_rot = (int64_t)rot.size();
+    cu->static_levels = offs;
+    cu->rot_levels.clear();
+    int64_t acc = cu->n_static;
+    cu->rot_levels.push_back(acc);
+    for (int c = 0; c < 5 * be.n_bcol; c++) {
+        acc += blade_class_size(be, c);
+        cu->rot_levels.push_back(acc);
+    }
+    CHK(cu->pts.reserve(cu->n_static + cu->n_rot, cu->tsize));
+    CHK(upload_points(cu, cu->pts, 0, hs));
+
+    cu->blade.n_blades = p->n_blades;
+    cu->blade.j_lo = (int32_t)p->blade_plane_lo;
+    cu->blade.n_planes = be.n_planes;
+    cu->blade.n_radial = be.n_radial;
+    cu->blade.r_inner = p->blade_inner_radius;
+    cu->blade.cx = p->center_x;
+    cu->blade.cz = p->center_z;
+    cu->blade.ghost = (int32_t)cu->prm.ghost;
+    cu->rp = *p;
+
+    // static occupancy bits: run the prepare kernel on an empty mask
+    if (!cu->static_bits && hipMalloc((void**)&cu->static_bits, cu->mask_bytes()) != hipSuccess)
+        return fail(TDLB_ERR_NOMEM, "static mask");
+    if (!cu->static_base && hipMalloc((void**)&cu->static_base, cu->mask_bytes()) != hipSuccess)
+        return fail(TDLB_ERR_NOMEM, "static mask");
+    HIPCHK(launch_zero(cu->static_bits, cu->mask_bytes(), cu->stream));
+    // prepare() zeroes F where the previous mask is clear: give it an all-ones mask
+    void* ones = nullptr;
+    if (hipMalloc(&ones, cu->mask_bytes()) != hipSuccess) return fail(TDLB_ERR_NOMEM, "mask scratch");
+    std::vector<uint64_t> allset(cu->mask_words, ~0ull);
+    hipError_t e = hipMemcpyAsync(ones, allset.data(), cu->mask_bytes(), hipMemcpyHostToDevice, cu->stream);
+    if (e == hipSuccess)
+        e = launch_force_prepare(cu->prec(), cu->pts.d, 0, cu->n_static, cu->F, (const uint64_t*)ones,
+                                 cu->static_bits, cu->g, cu->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(cu->stream);
+    (void)hipFree(ones);
+    if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("static mask: ") + hipGetErrorString(e));
+    {
+        std::vector<uint64_t> a(cu->mask_words), b(cu->mask_words);
+        HIPCHK(hipMemcpyAsync(a.data(), cu->static_bits, cu->mask_bytes(), hipMemcpyDeviceToHost, cu->stream));
+        HIPCHK(hipMemcpyAsync(b.data(), cu->gmask, cu->mask_bytes(), hipMemcpyDeviceToHost, cu->stream));
+        HIPCHK(hipStreamSynchronize(cu->stream));
+        for (size_t w = 0; w < a.size(); w++) a[w] |= b[w];
+        HIPCHK(hipMemcpyAsync(cu->static_base, a.data(), cu->mask_bytes(), hipMemcpyHostToDevice, cu->stream));
+        HIPCHK(hipStreamSynchronize(cu->stream));
+    }
+    cu->bound = true;
+    cu->alfa = alfa;
+    cu->beta = beta;
+
+    // initial forcing of main.cpp:305 on the (explicit) moments
+    CHK(gen_blades(cu, angle, delta));
+    CHK(bound_forcing(cu, MOMSRC_STORED, true));
+    return TDLB_OK;
+}
+
+int tdlb_cu_run(tdlb_cu_t cu, int64_t nsteps, uint64_t* step, double* angle) {
+    if (!cu || !step || !angle || nsteps < 0) return fail(TDLB_ERR_INVALID, "bad arguments");
+    if (!cu->bound) return fail(TDLB_ERR_STATE, "no geometry bound (tdlb_cu_bind_rushton)");
+    CHK(set_device(cu));
+    Rushton R(cu->rp);
+    const auto c = cu->cargs();
+    for (int64_t s = 0; s < nsteps; s++) {
+        *step += 1;                                             // running.incrementStep()
+        const double delta = to_prec(cu, R.step_increment(*step));
+        *angle += delta;                                        // running.angle += delta
+        CHK(gen_blades(cu, *angle, delta));                     // updateRotatingGeometry
+        const int c0 = prof_mark(cu);
+        if (cu->phase == PH_EXPLICIT) {
+            if (cu->simple()) {
+                // settle the ghost layer once (bounce-back rims), share it with the other lattice
+                HIPCHK(launch_bounce_back((int)cu->tsize, cu->prm.layout, cu->Q[cu->cur], cu->g, cu->stream));
+                HIPCHK(launch_ghost_copy((int)cu->tsize, cu->prm.layout, cu->Q[cu->cur], cu->Q[1 - cu->cur],
+                                         cu->g, cu->stream));
+                HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_COLL, cu->Q[cu->cur], cu->Q[cu->cur], cu->F,
+                                     cu->Nu, cu->fmask[cu->fm], cu->g, c, cu->stream));
+            } else {
+                HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_COLL, cu->even ? 1 : 0, cu->Q[0], cu->F,
+                                       cu->Nu, cu->fmask[cu->fm], cu->g, c, cu->stream));
+                cu->even = !cu->even;
+            }
+            cu->phase = PH_LAZY;
+        } else {
+            if (cu->simple()) {
+                HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_FUSED, cu->Q[cu->cur], cu->Q[1 - cu->cur],
+                                     cu->F, cu->Nu, cu->fmask[cu->fm], cu->g, c, cu->stream));
+                cu->cur = 1 - cu->cur;
+            } else {
+                HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_FUSED, cu->even ? 1 : 0, cu->Q[0], cu->F,
+                                       cu->Nu, cu->fmask[cu->fm], cu->g, c, cu->stream));
+                cu->even = !cu->even;
+            }
+        }
+        const int c1 = prof_mark(cu);
+        CHK(bound_forcing(cu, cu->simple() ? MOMSRC_PULL_POPS : MOMSRC_ESO_POPS, !cu->static_in_prev));
+        const int c2 = prof_mark(cu);
+        if (c0 >= 0 && c1 >= 0) cu->coll.push_back({c0, c1});
+        if (c1 >= 0 && c2 >= 0) cu->force.push_back({c1, c2});
+    }
+    return TDLB_OK;
+}
+
+int tdlb_cu_profile(tdlb_cu_t cu, int enable) {
+    if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    CHK(set_device(cu));
+    HIPCHK(hipStreamSynchronize(cu->stream));
+    cu->prof = enable != 0;
+    cu->ev_used = 0;
+    cu->coll.clear();
+    cu->force.clear();
+    return TDLB_OK;
+}
+
+int tdlb_cu_profile_read(tdlb_cu_t cu, double* collide_ms, int64_t* collide_launches, double* forcing_ms,
+                         int64_t* forcing_passes) {
+    if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    CHK(set_device(cu));
+    HIPCHK(hipStreamSynchronize(cu->stream));
+    auto sum = [&](const std::vector<std::pair<int, int>>& v, double* ms) -> int {
+        double t = 0.0;
+        for (auto& pr : v) {
+            float e = 0.f;
+            HIPCHK(hipEventElapsedTime(&e, cu->events[pr.first], cu->events[pr.second]));
+            t += e;
+        }
+        if (ms) *ms = t;
+        return TDLB_OK;
+    };
+    CHK(sum(cu->coll, collide_ms));
+    CHK(sum(cu->force, forcing_ms));
+    if (collide_launches) *collide_launches = (int64_t)cu->coll.size();
+    if (forcing_passes) *forcing_passes = (int64_t)cu->force.size();
+    return TDLB_OK;
+}
+
+int tdlb_cu_rotating_points(tdlb_cu_t cu, tdlb_ib_point* out, int64_t cap, int64_t* n) {
+    if (!cu || !n) return fail(TDLB_ERR_INVALID, "null argument");
+    if (!cu->bound) return fail(TDLB_ERR_STATE, "no geometry bound");
+    CHK(set_device(cu));
+    *n = cu->n_rot;
+    const int64_t m = std::min<int64_t>(cap, cu->n_rot);
+    if (!out || m <= 0) return TDLB_OK;
+    const size_t ts = cu->tsize;
+    HostPoints h;
+    h.resize(cu->n_rot, ts);
+    const int64_t o = cu->n_static;
+    HIPCHK(hipMemcpyAsync(h.i.data(), cu->pts.d.i + o, cu->n_rot * 4, hipMemcpyDeviceToHost, cu->stream));
+    HIPCHK(hipMemcpyAsync(h.j.data(), cu->pts.d.j + o, cu->n_rot * 4, hipMemcpyDeviceToHost, cu->stream));
+    HIPCHK(hipMemcpyAsync(h.k.data(), cu->pts.d.k + o, cu->n_rot * 4, hipMemcpyDeviceToHost, cu->stream));
+    void* src[] = {cu->pts.d.ifrac, cu->pts.d.kfrac, cu->pts.d.u, cu->pts.d.v, cu->pts.d.w};
+    std::vector<uint8_t>* dst[] = {&h.ifrac, &h.kfrac, &h.u, &h.v, &h.w};
+    for (int a = 0; a < 5; a++)
+        HIPCHK(hipMemcpyAsync(dst[a]->data(), (char*)src[a] + o * ts, cu->n_rot * ts, hipMemcpyDeviceToHost, cu->stream));
+    HIPCHK(hipStreamSynchronize(cu->stream));
+    auto get = [&](const std::vector<uint8_t>& v, int64_t q) -> double {
+        if (ts == 4) { float f; std::memcpy(&f, v.data() + q * 4, 4); return f; }
+        double d; std::memcpy(&d, v.data() + q * 8, 8); return d;
+    };
+    const int64_t gh = cu->prm.ghost;
+    for (int64_t q = 0; q < m; q++) {
+        out[q].i = h.i[q] - gh;
+        out[q].j = h.j[q] - gh;
+        out[q].k = h.k[q] - gh;
+        out[q].i_frac = get(h.ifrac, q);
+        out[q].k_frac = get(h.kfrac, q);
+        out[q].u_delta = get(h.u, q);
+        out[q].v_delta = get(h.v, q);
+        out[q].w_delta = get(h.w, q);
+    }
+    return TDLB_OK;
+}
+
+int tdlb_cu_solid_mask(tdlb_cu_t cu, uint8_t* out, size_t bytes) {
+    if (!cu || !out) return fail(TDLB_ERR_INVALID, "null argument");
+    if (!cu->bound) return fail(TDLB_ERR_STATE, "no geometry bound");
+    if (bytes != cu->g.size) return fail(TDLB_ERR_INVALID, "mask buffer must be xg*yg*zg bytes");
+    CHK(set_device(cu));
+    uint8_t* tmp = nullptr;
+    if (hipMalloc((void**)&tmp, bytes) != hipSuccess) return fail(TDLB_ERR_NOMEM, "mask staging");
+    hipError_t e = launch_solid_mask(cu->rp.center_x, cu->rp.center_z, cu->rp.tank_radius, tmp, cu->g,
+                                     (int)cu->prm.ghost, cu->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, bytes, hipMemcpyDeviceToHost, cu->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(cu->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("solid mask: ") + hipGetErrorString(e));
+    return TDLB_OK;
+}
+
+int tdlb_cu_forcing_info(tdlb_cu_t cu, int64_t* n_static, int64_t* n_rotating, int64_t* n_levels) {
+    if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    if (n_static) *n_static = cu->n_static;
+    if (n_rotating) *n_rotating = cu->n_rot;
+    if (n_levels)
+        *n_levels = (int64_t)(cu->static_levels.empty() ? 0 : cu->static_levels.size() - 1) +
+                    (int64_t)(cu->rot_levels.empty() ? 0 : cu->rot_levels.size() - 1);
+    return TDLB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,220 @@
+// Per-cell arithmetic of the Eggels-Somers scheme, host and device.
+//
+// moments:   tdlbcpp/src/CollisionEgglesSomers.hpp:185-256
+// collision: tdlbcpp/src/CollisionEgglesSomers.hpp:26-181 (+ velocity QVec.hpp:125-134)
+//
+// The reference is compiled with T = float and double literals, so several
+// sub-expressions are evaluated in double and rounded back to T.  To give
+// results identical to the reference, every expression below spells those
+// promotions out as (D) casts and keeps the reference's left-to-right order of
+// terms.  All code that includes this header is compiled with
+// -ffp-contract=off (no fused multiply-add).
+//
+// One deliberate rewrite: the reference divides each of the 18 relaxed moments
+// by 24.0 in double and rounds to float.  For T = float we multiply by the
+// double 1/24 instead; (float)((double)x * (1/24.0)) equals
+// (float)((double)x / 24.0) for every non-NaN float x (checked exhaustively over
+// all 2^32 bit patterns by tests/test_device_arith.py).
+#pragma once
+
+#include <cmath>
+#include "lattice.h"
+
+namespace tdlb {
+
+template <typename T>
+struct CollideParams {
+    T nu;      // kinematic viscosity (FlowParams::nu after calcNuAndRe_m)
+    T cs0;     // Smagorinsky constant
+    T g3;      // third-order relaxation
+    T b0, c0;  // 1/(1+6nu), 1-6nu in T, computed once per collision as the reference does
+    int use_les;
+};
+
+template <typename T>
+TDLB_HD inline CollideParams<T> make_collide_params(T nu, T cs0, T g3, int use_les) {
+    using D = double;
+    CollideParams<T> p;
+    p.nu = nu;
+    p.cs0 = cs0;
+    p.g3 = g3;
+    p.use_les = use_les;
+    p.b0 = (T)(1.0 / (1.0 + (D)(T(6) * nu)));
+    p.c0 = (T)(1.0 - (D)(T(6) * nu));
+    return p;
+}
+
+// sqrt(fabs(x)) in the precision of x (the reference's T = float build uses the
+// float overloads; both are correctly rounded on the host and on gfx950).
+TDLB_HD inline float sqrt_abs(float x) { return sqrtf(fabsf(x)); }
+TDLB_HD inline double sqrt_abs(double x) { return sqrt(fabs(x)); }
+
+template <typename T>
+TDLB_HD inline T div24(T x) {
+    return (T)((double)x / 24.0);
+}
+template <>
+TDLB_HD inline float div24<float>(float x) {
+    return (float)((double)x * (1.0 / 24.0));
+}
+
+// Eighteen moments of eighteen populations; same term order as the reference.
+template <typename T>
+TDLB_HD inline void es_moments(const T* f, T* m) {
+    using D = double;
+    const T two = T(2), three = T(3);
+    m[M01] = f[Q01] + f[Q03] + f[Q02] + f[Q04] + f[Q05] + f[Q06] + f[Q07] + f[Q14] + f[Q08] + f[Q13] + f[Q09]
+           + f[Q16] + f[Q10] + f[Q15] + f[Q11] + f[Q18] + f[Q12] + f[Q17];
+    m[M02] = f[Q01] - f[Q02] + f[Q07] - f[Q14] - f[Q08] + f[Q13] + f[Q09] - f[Q16] - f[Q10] + f[Q15];
+    m[M03] = f[Q03] - f[Q04] + f[Q07] + f[Q14] - f[Q08] - f[Q13] + f[Q11] - f[Q18] - f[Q12] + f[Q17];
+    m[M04] = f[Q05] - f[Q06] + f[Q09] + f[Q16] - f[Q10] - f[Q15] + f[Q11] + f[Q18] - f[Q12] - f[Q17];
+    m[M05] = -f[Q03] - f[Q04] - f[Q05] - f[Q06] + f[Q07] + f[Q14] + f[Q08] + f[Q13] + f[Q09] + f[Q16] + f[Q10]
+           + f[Q15];
+    m[M06] = f[Q07] - f[Q14] + f[Q08] - f[Q13];
+    m[M07] = -f[Q01] - f[Q02] - f[Q05] - f[Q06] + f[Q07] + f[Q14] + f[Q08] + f[Q13] + f[Q11] + f[Q18] + f[Q12]
+           + f[Q17];
+    m[M08] = f[Q09] - f[Q16] + f[Q10] - f[Q15];
+    m[M09] = f[Q11] - f[Q18] + f[Q12] - f[Q17];
+    m[M10] = -f[Q01] - f[Q03] - f[Q02] - f[Q04] + f[Q09] + f[Q16] + f[Q10] + f[Q15] + f[Q11] + f[Q18] + f[Q12]
+           + f[Q17];
+    m[M11] = -f[Q01] + f[Q02] + two * f[Q07] - two * f[Q14] - two * f[Q08] + two * f[Q13] - f[Q09] + f[Q16]
+           + f[Q10] - f[Q15];
+    m[M12] = -f[Q03] + f[Q04] + two * f[Q07] + two * f[Q14] - two * f[Q08] - two * f[Q13] - f[Q11] + f[Q18]
+           + f[Q12] - f[Q17];
+    m[M13] = -three * f[Q01] + three * f[Q02] + three * f[Q09] - three * f[Q16] - three * f[Q10]
+           + three * f[Q15];
+    m[M14] = -three * f[Q03] + three * f[Q04] + three * f[Q11] - three * f[Q18] - three * f[Q12]
+           + three * f[Q17];
+    m[M15] = -two * f[Q05] + two * f[Q06] + f[Q09] + f[Q16] - f[Q10] - f[Q15] + f[Q11] + f[Q18] - f[Q12]
+           - f[Q17];
+    m[M16] = -three * f[Q09] - three * f[Q16] + three * f[Q10] + three * f[Q15] + three * f[Q11]
+           + three * f[Q18] - three * f[Q12] - three * f[Q17];
+    // fourth order: double literals 0.5 / 1.5 make these sums double
+    const D h = 0.5, o = 1.5;
+    m[M17] = (T)(h * (D)f[Q01] + h * (D)f[Q03] + h * (D)f[Q02] + h * (D)f[Q04] - (D)f[Q05] - (D)f[Q06]
+                 - (D)f[Q07] - (D)f[Q14] - (D)f[Q08] - (D)f[Q13] + h * (D)f[Q09] + h * (D)f[Q16]
+                 + h * (D)f[Q10] + h * (D)f[Q15] + h * (D)f[Q11] + h * (D)f[Q18] + h * (D)f[Q12]
+                 + h * (D)f[Q17]);
+    m[M18] = (T)(o * (D)f[Q01] - o * (D)f[Q03] + o * (D)f[Q02] - o * (D)f[Q04] - o * (D)f[Q09]
+                 - o * (D)f[Q16] - o * (D)f[Q10] - o * (D)f[Q15] + o * (D)f[Q11] + o * (D)f[Q18]
+                 + o * (D)f[Q12] + o * (D)f[Q17]);
+}
+
+// Only density and momentum (what IB forcing reads), same arithmetic as es_moments.
+template <typename T>
+TDLB_HD inline void es_rho_j(const T* f, T& rho, T& jx, T& jy, T& jz) {
+    rho = f[Q01] + f[Q03] + f[Q02] + f[Q04] + f[Q05] + f[Q06] + f[Q07] + f[Q14] + f[Q08] + f[Q13] + f[Q09]
+        + f[Q16] + f[Q10] + f[Q15] + f[Q11] + f[Q18] + f[Q12] + f[Q17];
+    jx = f[Q01] - f[Q02] + f[Q07] - f[Q14] - f[Q08] + f[Q13] + f[Q09] - f[Q16] - f[Q10] + f[Q15];
+    jy = f[Q03] - f[Q04] + f[Q07] + f[Q14] - f[Q08] - f[Q13] + f[Q11] - f[Q18] - f[Q12] + f[Q17];
+    jz = f[Q05] - f[Q06] + f[Q09] + f[Q16] - f[Q10] - f[Q15] + f[Q11] + f[Q18] - f[Q12] - f[Q17];
+}
+
+// Collision of one cell: m (moments) in, p (post-collision populations) out.
+// nuc is the cell's eddy viscosity: read (previous step) and written with LES.
+template <typename T>
+TDLB_HD inline void es_collide(const T* m, T fx, T fy, T fz, T& nuc, const CollideParams<T>& P, T* p) {
+    using D = double;
+    const D inv_rho = 1.0 / (D)m[M01];
+    const T ux = (T)(inv_rho * ((D)m[M02] + 0.5 * (D)fx));
+    const T uy = (T)(inv_rho * ((D)m[M03] + 0.5 * (D)fy));
+    const T uz = (T)(inv_rho * ((D)m[M04] + 0.5 * (D)fz));
+
+    T b = P.b0, c = P.c0;
+    if (P.use_les) {
+        const T fct = (T)(3.0 / ((D)m[M01] * (1.0 + 6.0 * (D)(nuc + P.nu))));
+        const D hfx = 0.5 * (D)fx, hfy = 0.5 * (D)fy, hfz = 0.5 * (D)fz;
+        const T dudx = (T)((D)fct * (((D)m[M02] + hfx * (D)ux) - (D)m[M05]));
+        const T dvdy = (T)((D)fct * (((D)m[M03] + hfy * (D)uy) - (D)m[M07]));
+        const T dwdz = (T)((D)fct * (((D)m[M04] + hfz * (D)uz) - (D)m[M10]));
+        const T divv = dudx + dvdy + dwdz;
+        const T fct2 = T(2) * fct;
+        const T sxy = (T)((D)fct2 * (((D)m[M03] + hfy * (D)ux) - (D)m[M06]));
+        const T sxz = (T)((D)fct2 * (((D)m[M04] + hfz * (D)ux) - (D)m[M08]));
+        const T syz = (T)((D)fct2 * (((D)m[M04] + hfz * (D)uy) - (D)m[M09]));
+        const T sh_t = T(2) * dudx * dudx + T(2) * dvdy * dvdy + T(2) * dwdz * dwdz + sxy * sxy + sxz * sxz
+                     + syz * syz;
+        const T sh = (T)((D)sh_t - (2.0 / 3.0) * (D)divv * (D)divv);
+        nuc = P.cs0 * P.cs0 * sqrt_abs(sh);
+        const T nut = nuc + P.nu;
+        b = (T)(1.0 / (1.0 + (D)(T(6) * nut)));
+        c = (T)(1.0 - (D)(T(6) * nut));
+    }
+
+    const D jx2 = 2.0 * ((D)m[M02] + 0.5 * (D)fx);
+    const D jy2 = 2.0 * ((D)m[M03] + 0.5 * (D)fy);
+    const D jz2 = 2.0 * ((D)m[M04] + 0.5 * (D)fz);
+    const D bd = (D)b;
+
+    T a[kQ];
+    a[M01] = m[M01];
+    a[M02] = m[M02] + fx;
+    a[M03] = m[M03] + fy;
+    a[M04] = m[M04] + fz;
+    a[M05] = (T)((jx2 * (D)ux - (D)(m[M05] * c)) * bd);
+    a[M06] = (T)((jx2 * (D)uy - (D)(m[M06] * c)) * bd);
+    a[M07] = (T)((jy2 * (D)uy - (D)(m[M07] * c)) * bd);
+    a[M08] = (T)((jx2 * (D)uz - (D)(m[M08] * c)) * bd);
+    a[M09] = (T)((jy2 * (D)uz - (D)(m[M09] * c)) * bd);
+    a[M10] = (T)((jz2 * (D)uz - (D)(m[M10] * c)) * bd);
+    const T ng3 = -P.g3;
+#pragma unroll
+    for (int l = M11; l <= M16; l++) a[l] = ng3 * m[l];
+    a[M17] = T(0.0);
+    a[M18] = T(0.0);
+#pragma unroll
+    for (int l = 0; l < kQ; l++) a[l] = div24(a[l]);
+
+    const T t2 = T(2), t3 = T(3), t4 = T(4), t6 = T(6);
+    const D o = 1.5;
+    // face populations: integer coefficients, all in T
+    p[Q01] = t2 * a[M01] + t4 * a[M02] + t3 * a[M05] - t3 * a[M07] - t3 * a[M10] - t2 * a[M11] - t2 * a[M13]
+           + t2 * a[M17] + t2 * a[M18];
+    p[Q02] = t2 * a[M01] - t4 * a[M02] + t3 * a[M05] - t3 * a[M07] - t3 * a[M10] + t2 * a[M11] + t2 * a[M13]
+           + t2 * a[M17] + t2 * a[M18];
+    p[Q03] = t2 * a[M01] + t4 * a[M03] - t3 * a[M05] + t3 * a[M07] - t3 * a[M10] - t2 * a[M12] - t2 * a[M14]
+           + t2 * a[M17] - t2 * a[M18];
+    p[Q04] = t2 * a[M01] - t4 * a[M03] - t3 * a[M05] + t3 * a[M07] - t3 * a[M10] + t2 * a[M12] + t2 * a[M14]
+           + t2 * a[M17] - t2 * a[M18];
+    p[Q05] = t2 * a[M01] + t4 * a[M04] - t3 * a[M05] - t3 * a[M07] + t3 * a[M10] - t4 * a[M15] - t4 * a[M17];
+    p[Q06] = t2 * a[M01] - t4 * a[M04] - t3 * a[M05] - t3 * a[M07] + t3 * a[M10] + t4 * a[M15] - t4 * a[M17];
+    // edge populations: the first 1.5 coefficient turns the rest of the sum into double
+    p[Q07] = (T)((D)(a[M01] + t2 * a[M02] + t2 * a[M03]) + o * (D)a[M05] + (D)(t6 * a[M06]) + o * (D)a[M07]
+                 - o * (D)a[M10] + (D)(t2 * a[M11]) + (D)(t2 * a[M12]) - (D)(t2 * a[M17]));
+    p[Q14] = (T)((D)(a[M01] - t2 * a[M02] + t2 * a[M03]) + o * (D)a[M05] - (D)(t6 * a[M06]) + o * (D)a[M07]
+                 - o * (D)a[M10] - (D)(t2 * a[M11]) + (D)(t2 * a[M12]) - (D)(t2 * a[M17]));
+    p[Q08] = (T)((D)(a[M01] - t2 * a[M02] - t2 * a[M03]) + o * (D)a[M05] + (D)(t6 * a[M06]) + o * (D)a[M07]
+                 - o * (D)a[M10] - (D)(t2 * a[M11]) - (D)(t2 * a[M12]) - (D)(t2 * a[M17]));
+    p[Q13] = (T)((D)(a[M01] + t2 * a[M02] - t2 * a[M03]) + o * (D)a[M05] - (D)(t6 * a[M06]) + o * (D)a[M07]
+                 - o * (D)a[M10] + (D)(t2 * a[M11]) - (D)(t2 * a[M12]) - (D)(t2 * a[M17]));
+    p[Q09] = (T)((D)(a[M01] + t2 * a[M02] + t2 * a[M04]) + o * (D)a[M05] - o * (D)a[M07] + (D)(t6 * a[M08])
+                 + o * (D)a[M10] - (D)a[M11] + (D)a[M13] + (D)a[M15] - (D)a[M16] + (D)a[M17] - (D)a[M18]);
+    p[Q16] = (T)((D)(a[M01] - t2 * a[M02] + t2 * a[M04]) + o * (D)a[M05] - o * (D)a[M07] - (D)(t6 * a[M08])
+                 + o * (D)a[M10] + (D)a[M11] - (D)a[M13] + (D)a[M15] - (D)a[M16] + (D)a[M17] - (D)a[M18]);
+    p[Q10] = (T)((D)(a[M01] - t2 * a[M02] - t2 * a[M04]) + o * (D)a[M05] - o * (D)a[M07] + (D)(t6 * a[M08])
+                 + o * (D)a[M10] + (D)a[M11] - (D)a[M13] - (D)a[M15] + (D)a[M16] + (D)a[M17] - (D)a[M18]);
+    p[Q15] = (T)((D)(a[M01] + t2 * a[M02] - t2 * a[M04]) + o * (D)a[M05] - o * (D)a[M07] - (D)(t6 * a[M08])
+                 + o * (D)a[M10] - (D)a[M11] + (D)a[M13] - (D)a[M15] + (D)a[M16] + (D)a[M17] - (D)a[M18]);
+    p[Q11] = (T)((D)(a[M01] + t2 * a[M03] + t2 * a[M04]) - o * (D)a[M05] + o * (D)a[M07] + (D)(t6 * a[M09])
+                 + o * (D)a[M10] - (D)a[M12] + (D)a[M14] + (D)a[M15] + (D)a[M16] + (D)a[M17] + (D)a[M18]);
+    p[Q18] = (T)((D)(a[M01] - t2 * a[M03] + t2 * a[M04]) - o * (D)a[M05] + o * (D)a[M07] - (D)(t6 * a[M09])
+                 + o * (D)a[M10] + (D)a[M12] - (D)a[M14] + (D)a[M15] + (D)a[M16] + (D)a[M17] + (D)a[M18]);
+    p[Q12] = (T)((D)(a[M01] - t2 * a[M03] - t2 * a[M04]) - o * (D)a[M05] + o * (D)a[M07] + (D)(t6 * a[M09])
+                 + o * (D)a[M10] + (D)a[M12] - (D)a[M14] - (D)a[M15] - (D)a[M16] + (D)a[M17] + (D)a[M18]);
+    p[Q17] = (T)((D)(a[M01] + t2 * a[M03] - t2 * a[M04]) - o * (D)a[M05] + o * (D)a[M07] - (D)(t6 * a[M09])
+                 + o * (D)a[M10] - (D)a[M12] + (D)a[M14] - (D)a[M15] - (D)a[M16] + (D)a[M17] + (D)a[M18]);
+}
+
+// Smoothed-delta weight per axis (tdlbcpp/src/Forcing.hpp:18-35), evaluated in double.
+template <typename T>
+TDLB_HD inline T ib_weight(T xin) {
+    T w = T(0.);
+    const double x = (double)(xin < T(0) ? -xin : xin);
+    if (x <= 1.5) {
+        if (x <= 0.5) w = (T)((1.0 / 3.0) * (1.0 + sqrt(-3.0 * x * x + 1.0)));
+        else w = (T)((1.0 / 6.0) * (5.0 - (3.0 * x) - sqrt((-3.0 * ((1.0 - x) * (1.0 - x))) + 1.0)));
+    }
+    return w;
+}
+
+}  // namespace tdlb

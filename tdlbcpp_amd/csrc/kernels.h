@@ -1,0 +1,88 @@
+// Host-side launchers of the gfx950 kernels (implemented in kernels.hip).
+// Pointers are device pointers; every launcher enqueues on `s` and returns the
+// launch status.  prec: 0 float, 1 double.  layout: LayoutIJKL/LayoutLIJK.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "device.h"
+
+namespace tdlb {
+
+enum StepMode : int {
+    MODE_COLL = 0,      // moments in Q -> collide -> populations (ComputeUnit::collision)
+    MODE_MOM = 1,       // populations in Q -> moments (ComputeUnit::moments)
+    MODE_FUSED = 2,     // streamed populations -> moments -> collide -> populations
+    MODE_MOM_PULL = 3,  // Simple only: pull-stream from src -> moments into dst
+};
+
+// How IB forcing obtains density and momentum of a stencil cell.
+enum MomSource : int {
+    MOMSRC_STORED = 0,     // Q holds moments (read through the streaming accessor)
+    MOMSRC_ESO_POPS = 1,   // Esotwist: Q holds streamed populations at the current parity
+    MOMSRC_PULL_POPS = 2,  // Simple: Q holds post-collision populations, stream by pull
+};
+
+struct CollideArgs {
+    double nu, cs0, g3;  // converted to the CU's type on the device side
+    int use_les;
+};
+
+// IB points on the device, structure of arrays in the CU's type.
+struct PointsDev {
+    int32_t* i;  // cell index including the ghost offset
+    int32_t* j;
+    int32_t* k;
+    void* ifrac;  // T arrays
+    void* kfrac;
+    void* u;
+    void* v;
+    void* w;
+};
+
+// Blade layout for on-device generation of the rotating (updating) IB points.
+struct BladeLayout {
+    int32_t n_blades;
+    int32_t j_lo, n_planes;  // first plane (pre-ghost j) and number of planes
+    int32_t n_radial;        // points per blade per plane
+    double r_inner;          // radius of the first point; spacing 1
+    double cx, cz;           // axis position (pre-ghost grid coordinates)
+    int32_t ghost;
+};
+struct BladeStep {
+    double cosb[16], sinb[16];  // per-blade orientation this step
+    double omega;               // angular increment this step (rad / step)
+};
+
+hipError_t launch_fill_q(int prec, int layout, void* Q, const Grid& g, double rho, hipStream_t s);
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+hipError_t launch_esotwist(int prec, int layout, int mode, int even, void* Q, const void* F, void* Nu,
+                           const uint64_t* fmask, const Grid& g, const CollideArgs& c, hipStream_t s);
+hipError_t launch_simple(int prec, int layout, int mode, const void* src, void* dst, const void* F, void* Nu,
+                         const uint64_t* fmask, const Grid& g, const CollideArgs& c, hipStream_t s);
+
+// element size 4 or 8: pure data movement on every cell
+hipError_t launch_push_all(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s);
+hipError_t launch_pull_all(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s);
+hipError_t launch_bounce_back(int esize, int layout, void* Q, const Grid& g, hipStream_t s);
+hipError_t launch_ghost_fill(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s);
+hipError_t launch_ghost_copy(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s);
+
+hipError_t launch_mask_from_f(int prec, const void* F, uint64_t* mask, uint64_t* gmask, const Grid& g,
+                              hipStream_t s);
+hipError_t launch_masked_f(int prec, const void* F, const uint64_t* mask, void* out, const Grid& g, hipStream_t s);
+hipError_t launch_zero_interior_o(uint8_t* O, const Grid& g, hipStream_t s);
+
+hipError_t launch_force_prepare(int prec, const PointsDev& p, int64_t begin, int64_t count, void* F,
+                                const uint64_t* prev, uint64_t* next, const Grid& g, hipStream_t s);
+hipError_t launch_force_level(int prec, int layout, int streaming, int momsrc, int even, const PointsDev& p,
+                              int64_t begin, int64_t count, void* F, const void* Q, const Grid& g, double alfa,
+                              double beta, hipStream_t s);
+hipError_t launch_gen_blades(int prec, const BladeLayout& bl, const BladeStep& st, const PointsDev& out,
+                             int64_t offset, hipStream_t s);
+hipError_t launch_solid_mask(double cx, double cz, double radius, uint8_t* out, const Grid& g, int ghost,
+                             hipStream_t s);
+
+}  // namespace tdlb

@@ -1,0 +1,843 @@
+// gfx950 kernels of the 18-velocity Eggels-Somers ComputeUnit.
+//
+// Mapping: one thread per interior cell, k (the fastest index of both layouts)
+// on the lanes, one block per run of k in one (i,j) row.  In LIJK every slot
+// load and store of a wavefront is a contiguous 256 B run; every memory
+// location is read once and written once per step, so the kernels move the
+// algorithmic 2 x 18 x sizeof(T) bytes per cell (+ Nu with LES).
+//
+// Reference semantics (file:line in the reference tree):
+//   collision  CollisionEgglesSomers.hpp:26-181   moments  :185-256
+//   Esotwist accessors  StreamingEsoteric.hpp:27-180
+//   Simple push/pull    StreamingSimple.hpp:18-126   bounce-back Boundary.hpp:13-250
+//   forcing             Forcing.hpp:56-176
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+#include "es_collide.h"
+#include "kernels.h"
+#include "rushton.h"
+
+namespace tdlb {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline uint32_t blocks_for(uint64_t n, int bs = kBlock) {
+    uint64_t b = (n + bs - 1) / bs;
+    return (uint32_t)(b > 0x7fffffffull ? 0x7fffffffull : (b ? b : 1));
+}
+
+template <typename T>
+__device__ __forceinline__ T ldT(const void* p, uint64_t a) {
+    return static_cast<const T*>(p)[a];
+}
+
+// ------------------------------------------------------------------ fills
+template <typename T, int L>
+__global__ void k_fill_q(T* __restrict__ Q, Grid g, T rho) {
+    const uint64_t total = g.size * (uint64_t)kQ;
+    for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < total;
+         a += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t slot = (L == LayoutIJKL) ? a % kQ : a / g.size;
+        Q[a] = slot == 0 ? rho : T(0);
+    }
+}
+
+__global__ void k_zero_words(uint32_t* __restrict__ p, uint64_t n) {
+    for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < n; a += (uint64_t)gridDim.x * blockDim.x)
+        p[a] = 0u;
+}
+__global__ void k_zero_bytes(uint8_t* __restrict__ p, uint64_t n) {
+    for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < n; a += (uint64_t)gridDim.x * blockDim.x)
+        p[a] = 0;
+}
+__global__ void k_copy_words(uint32_t* __restrict__ d, const uint32_t* __restrict__ s, uint64_t n) {
+    for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < n; a += (uint64_t)gridDim.x * blockDim.x)
+        d[a] = s[a];
+}
+__global__ void k_copy_bytes(uint8_t* __restrict__ d, const uint8_t* __restrict__ s, uint64_t n) {
+    for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < n; a += (uint64_t)gridDim.x * blockDim.x)
+        d[a] = s[a];
+}
+
+// ------------------------------------------------------------------ Esotwist
+// Direction d of cell c is stored at home(c,d) = c + (1 where e_d < 0, periodic
+// on the interior), in slot d at even parity and opp(d) at odd parity.  read
+// takes the 18 values from there; write puts population d where the read of
+// opp(d) looks (StreamingEsoteric.hpp:82-130); writeMoments puts value d back
+// where d was read.
+template <typename T, int L, int MODE, int EVEN>
+__global__ __launch_bounds__(kBlock) void k_esotwist(T* __restrict__ Q, const T* __restrict__ F,
+                                                     T* __restrict__ Nu, const uint64_t* __restrict__ fmask,
+                                                     Grid g, CollideParams<T> P, uint32_t nkb) {
+    const RowBlock rb = row_block(g, blockIdx.x, nkb, blockDim.x);
+    const int i = rb.i, j = rb.j, k = rb.k0 + (int)threadIdx.x;
+    if (k > g.zg1) return;
+    const uint32_t n = cell_index(g, i, j, k);
+    const int32_t di = (i == g.xg1) ? -(int32_t)((uint32_t)(g.xg1 - 1) * g.yz) : (int32_t)g.yz;
+    const int32_t dj = (j == g.yg1) ? -(g.yg1 - 1) * g.zg : g.zg;
+    const int32_t dk = (k == g.zg1) ? -(g.zg1 - 1) : 1;
+
+    uint32_t home[kQ];
+#pragma unroll
+    for (int d = 0; d < kQ; d++)
+        home[d] = n + (uint32_t)((twist_dx(d) ? di : 0) + (twist_dy(d) ? dj : 0) + (twist_dz(d) ? dk : 0));
+
+    T v[kQ];
+#pragma unroll
+    for (int d = 0; d < kQ; d++) v[d] = Q[qaddr<L>(g, home[d], EVEN ? d : opposite(d))];
+
+    if (MODE == MODE_MOM) {
+        T m[kQ];
+        es_moments(v, m);
+#pragma unroll
+        for (int d = 0; d < kQ; d++) Q[qaddr<L>(g, home[d], EVEN ? d : opposite(d))] = m[d];
+        return;
+    }
+
+    T fx = T(0), fy = T(0), fz = T(0);
+    if (mask_test(fmask, g, i, j, k)) {
+        fx = F[3 * (uint64_t)n + 0];
+        fy = F[3 * (uint64_t)n + 1];
+        fz = F[3 * (uint64_t)n + 2];
+    }
+    T nuc = P.use_les ? Nu[n] : T(0);
+    T p[kQ];
+    if (MODE == MODE_FUSED) {
+        T m[kQ];
+        es_moments(v, m);
+        es_collide(m, fx, fy, fz, nuc, P, p);
+    } else {
+        es_collide(v, fx, fy, fz, nuc, P, p);
+    }
+    if (P.use_les) Nu[n] = nuc;
+#pragma unroll
+    for (int d = 0; d < kQ; d++) {
+        const int e = opposite(d);
+        Q[qaddr<L>(g, home[e], EVEN ? e : opposite(e))] = p[d];
+    }
+}
+
+// ------------------------------------------------------------------ Simple
+// In-place collide / moments, or pull from src (the two-lattice form of the
+// reference's in-place push: slot d of an interior cell c takes src[c-e_d][d]
+// when c-e_d is interior and keeps src[c][d] otherwise).
+template <typename T, int L, int MODE>
+__global__ __launch_bounds__(kBlock) void k_simple(const T* src, T* dst,  // equal for in-place modes
+                                                   const T* __restrict__ F, T* __restrict__ Nu,
+                                                   const uint64_t* __restrict__ fmask, Grid g,
+                                                   CollideParams<T> P, uint32_t nkb) {
+    const RowBlock rb = row_block(g, blockIdx.x, nkb, blockDim.x);
+    const int i = rb.i, j = rb.j, k = rb.k0 + (int)threadIdx.x;
+    if (k > g.zg1) return;
+    const uint32_t n = cell_index(g, i, j, k);
+
+    T v[kQ];
+    if (MODE == MODE_FUSED || MODE == MODE_MOM_PULL) {
+        const bool im = i >= 2, ip = i <= g.xg1 - 1;  // cell i-1 / i+1 interior
+        const bool jm = j >= 2, jp = j <= g.yg1 - 1;
+        const bool km = k >= 2, kp = k <= g.zg1 - 1;
+#pragma unroll
+        for (int d = 0; d < kQ; d++) {
+            const int ex = dir_x(d), ey = dir_y(d), ez = dir_z(d);
+            const bool ok = (ex > 0 ? im : ex < 0 ? ip : true) && (ey > 0 ? jm : ey < 0 ? jp : true) &&
+                            (ez > 0 ? km : ez < 0 ? kp : true);
+            const int32_t off = -(ex * (int32_t)g.yz + ey * g.zg + ez);
+            v[d] = src[qaddr<L>(g, ok ? n + (uint32_t)off : n, d)];
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < kQ; d++) v[d] = src[qaddr<L>(g, n, d)];
+    }
+
+    if (MODE == MODE_MOM || MODE == MODE_MOM_PULL) {
+        T m[kQ];
+        es_moments(v, m);
+#pragma unroll
+        for (int d = 0; d < kQ; d++) dst[qaddr<L>(g, n, d)] = m[d];
+        return;
+    }
+
+    T fx = T(0), fy = T(0), fz = T(0);
+    if (mask_test(fmask, g, i, j, k)) {
+        fx = F[3 * (uint64_t)n + 0];
+        fy = F[3 * (uint64_t)n + 1];
+        fz = F[3 * (uint64_t)n + 2];
+    }
+    T nuc = P.use_les ? Nu[n] : T(0);
+    T p[kQ];
+    if (MODE == MODE_FUSED) {
+        T m[kQ];
+        es_moments(v, m);
+        es_collide(m, fx, fy, fz, nuc, P, p);
+    } else {
+        es_collide(v, fx, fy, fz, nuc, P, p);
+    }
+    if (P.use_les) Nu[n] = nuc;
+#pragma unroll
+    for (int d = 0; d < kQ; d++) dst[qaddr<L>(g, n, d)] = p[d];
+}
+
+// ------------------------------------------------------------------ data movement on every cell
+__device__ __forceinline__ void decode_cell(const Grid& g, uint64_t n, int& i, int& j, int& k) {
+    i = (int)(n / g.yz);
+    const uint32_t r = (uint32_t)(n - (uint64_t)i * g.yz);
+    j = (int)(r / (uint32_t)g.zg);
+    k = (int)(r - (uint32_t)j * (uint32_t)g.zg);
+}
+
+// push: dst[c][d] = src[c-e_d][d] if c-e_d is interior, else src[c][d] (all cells)
+template <typename W, int L>
+__global__ void k_push_all(const W* __restrict__ src, W* __restrict__ dst, Grid g) {
+    for (uint64_t n = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; n < g.size;
+         n += (uint64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        decode_cell(g, n, i, j, k);
+#pragma unroll
+        for (int d = 0; d < kQ; d++) {
+            const int si = i - dir_x(d), sj = j - dir_y(d), sk = k - dir_z(d);
+            const uint32_t s = is_interior(g, si, sj, sk) ? cell_index(g, si, sj, sk) : (uint32_t)n;
+            dst[qaddr<L>(g, (uint32_t)n, d)] = src[qaddr<L>(g, s, d)];
+        }
+    }
+}
+
+// pull: interior dst[c][d] = src[c-e_d][d]; ghosts copied
+template <typename W, int L>
+__global__ void k_pull_all(const W* __restrict__ src, W* __restrict__ dst, Grid g) {
+    for (uint64_t n = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; n < g.size;
+         n += (uint64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        decode_cell(g, n, i, j, k);
+        const bool in = is_interior(g, i, j, k);
+#pragma unroll
+        for (int d = 0; d < kQ; d++) {
+            const uint32_t s = in ? cell_index(g, i - dir_x(d), j - dir_y(d), k - dir_z(d)) : (uint32_t)n;
+            dst[qaddr<L>(g, (uint32_t)n, d)] = src[qaddr<L>(g, s, d)];
+        }
+    }
+}
+
+// One face of the bounce-back, in place: ghost slot d (5 inward directions) of a
+// face cell takes slot opp(d) of the neighbour g+e_d.  Faces run in the
+// reference's order as separate launches because face rims read ghosts of
+// faces handled later.
+template <typename W, int L>
+__global__ void k_bb_face(W* __restrict__ Q, Grid g, int face) {
+    const int axis = face >> 1;        // 0 i, 1 j, 2 k
+    const int inward = (face & 1) ? -1 : 1;
+    const int pos = (face & 1) ? (axis == 0 ? g.xg1 + 1 : axis == 1 ? g.yg1 + 1 : g.zg1 + 1) : 0;
+    const int na = axis == 0 ? g.yg1 : g.xg1;                 // first free axis extent
+    const int nb = axis == 2 ? g.yg1 : g.zg1;                 // second free axis extent
+    const uint64_t total = (uint64_t)na * nb;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const int a = (int)(t / nb) + 1, b = (int)(t % nb) + 1;
+        int i, j, k;
+        if (axis == 0) { i = pos; j = a; k = b; }
+        else if (axis == 1) { i = a; j = pos; k = b; }
+        else { i = a; j = b; k = pos; }
+        const uint32_t c = cell_index(g, i, j, k);
+#pragma unroll
+        for (int d = 0; d < kQ; d++) {
+            const int comp = axis == 0 ? dir_x(d) : axis == 1 ? dir_y(d) : dir_z(d);
+            if (comp != inward) continue;
+            const uint32_t s = cell_index(g, i + dir_x(d), j + dir_y(d), k + dir_z(d));
+            Q[qaddr<L>(g, c, d)] = Q[qaddr<L>(g, s, opposite(d))];
+        }
+    }
+}
+
+// The 12 edge lines of the bounce-back (Boundary.hpp:41-132): on an edge the
+// one direction pointing inward on both boundary axes takes opp(d) of g+e_d.
+template <typename W, int L>
+__global__ void k_bb_edges(W* __restrict__ Q, Grid g) {
+    const int nmax = max(g.xg1, max(g.yg1, g.zg1));
+    const uint64_t total = 12ull * nmax;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const int line = (int)(t / nmax), s = (int)(t % nmax) + 1;
+        const int X0 = g.xg1 + 1, Y0 = g.yg1 + 1, Z0 = g.zg1 + 1;
+        int i, j, k, d;
+        switch (line) {
+            case 0: i = 0; j = 0; k = s; d = Q07; break;
+            case 1: i = X0; j = 0; k = s; d = Q14; break;
+            case 2: i = 0; j = Y0; k = s; d = Q13; break;
+            case 3: i = X0; j = Y0; k = s; d = Q08; break;
+            case 4: i = 0; k = 0; j = s; d = Q09; break;
+            case 5: i = 0; k = Z0; j = s; d = Q15; break;
+            case 6: i = X0; k = Z0; j = s; d = Q10; break;
+            case 7: i = X0; k = 0; j = s; d = Q16; break;
+            case 8: j = 0; k = 0; i = s; d = Q11; break;
+            case 9: j = 0; k = Z0; i = s; d = Q17; break;
+            case 10: j = Y0; k = Z0; i = s; d = Q12; break;
+            default: j = Y0; k = 0; i = s; d = Q18; break;
+        }
+        const int lim = line < 4 ? g.zg1 : line < 8 ? g.yg1 : g.xg1;
+        if (s > lim) continue;
+        const uint32_t c = cell_index(g, i, j, k);
+        const uint32_t src = cell_index(g, i + dir_x(d), j + dir_y(d), k + dir_z(d));
+        Q[qaddr<L>(g, c, d)] = Q[qaddr<L>(g, src, opposite(d))];
+    }
+}
+
+// Ghost cells enumerated face by face without repeats:
+// i = 0 and i = xg0 planes, then j = 0 / yg0 with 1 <= i <= xg1, then k = 0 / zg0
+// with 1 <= i <= xg1, 1 <= j <= yg1.
+__device__ __forceinline__ bool ghost_cell(const Grid& g, uint64_t t, int& i, int& j, int& k) {
+    const uint64_t A = (uint64_t)g.yz;                       // one i-plane
+    const uint64_t B = (uint64_t)g.xg1 * g.zg;               // one j-plane without i ghosts
+    const uint64_t C = (uint64_t)g.xg1 * g.yg1;              // one k-plane without i, j ghosts
+    if (t < 2 * A) {
+        i = t < A ? 0 : g.xg1 + 1;
+        const uint64_t r = t % A;
+        j = (int)(r / g.zg);
+        k = (int)(r % g.zg);
+        return true;
+    }
+    t -= 2 * A;
+    if (t < 2 * B) {
+        j = t < B ? 0 : g.yg1 + 1;
+        const uint64_t r = t % B;
+        i = (int)(r / g.zg) + 1;
+        k = (int)(r % g.zg);
+        return true;
+    }
+    t -= 2 * B;
+    if (t < 2 * C) {
+        k = t < C ? 0 : g.zg1 + 1;
+        const uint64_t r = t % C;
+        i = (int)(r / g.yg1) + 1;
+        j = (int)(r % g.yg1) + 1;
+        return true;
+    }
+    return false;
+}
+__host__ __device__ inline uint64_t ghost_count(const Grid& g) {
+    return 2ull * g.yz + 2ull * (uint64_t)g.xg1 * g.zg + 2ull * (uint64_t)g.xg1 * g.yg1;
+}
+
+// Ghost slots the reference rewrites every step from interior values: the push
+// into the ghost layer (c-e_d interior) and the bounce-back slots whose source
+// g+e_d is interior.  Rim bounce-back slots (ghost source) and never-written
+// slots are left as they are in dst.
+template <typename W, int L>
+__global__ void k_ghost_fill(const W* __restrict__ src, W* __restrict__ dst, Grid g) {
+    const uint64_t total = ghost_count(g);
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        if (!ghost_cell(g, t, i, j, k)) continue;
+        const int bi = (i == 0) ? 1 : (i == g.xg1 + 1) ? -1 : 0;  // inward sign per boundary axis
+        const int bj = (j == 0) ? 1 : (j == g.yg1 + 1) ? -1 : 0;
+        const int bk = (k == 0) ? 1 : (k == g.zg1 + 1) ? -1 : 0;
+        const int nb = (bi != 0) + (bj != 0) + (bk != 0);
+        const uint32_t c = cell_index(g, i, j, k);
+#pragma unroll
+        for (int d = 0; d < kQ; d++) {
+            const int ex = dir_x(d), ey = dir_y(d), ez = dir_z(d);
+            const int si = i - ex, sj = j - ey, sk = k - ez;
+            if (is_interior(g, si, sj, sk)) {
+                dst[qaddr<L>(g, c, d)] = src[qaddr<L>(g, cell_index(g, si, sj, sk), d)];
+                continue;
+            }
+            bool bb = false;
+            if (nb == 1) {
+                bb = (bi != 0 && ex == bi) || (bj != 0 && ey == bj) || (bk != 0 && ez == bk);
+            } else if (nb == 2) {
+                bb = (bi == 0 || ex == bi) && (bj == 0 || ey == bj) && (bk == 0 || ez == bk) &&
+                     (bi != 0 || ex == 0) && (bj != 0 || ey == 0) && (bk != 0 || ez == 0);
+            }
+            if (!bb) continue;
+            const int ti = i + ex, tj = j + ey, tk = k + ez;
+            if (!is_interior(g, ti, tj, tk)) continue;
+            dst[qaddr<L>(g, c, d)] = src[qaddr<L>(g, cell_index(g, ti, tj, tk), opposite(d))];
+        }
+    }
+}
+
+template <typename W, int L>
+__global__ void k_ghost_copy(const W* __restrict__ src, W* __restrict__ dst, Grid g) {
+    const uint64_t total = ghost_count(g);
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        if (!ghost_cell(g, t, i, j, k)) continue;
+        const uint32_t c = cell_index(g, i, j, k);
+#pragma unroll
+        for (int d = 0; d < kQ; d++) dst[qaddr<L>(g, c, d)] = src[qaddr<L>(g, c, d)];
+    }
+}
+
+// ------------------------------------------------------------------ F occupancy
+// Bit per cell: any component of F[c] with a non-zero bit pattern.  gmask gets
+// the ghost cells only.
+template <typename T>
+__global__ void k_mask_from_f(const T* __restrict__ F, uint64_t* __restrict__ mask, uint64_t* __restrict__ gmask,
+                              Grid g) {
+    const uint64_t words = (uint64_t)g.xg * g.yg * g.mask_w;
+    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < words;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t row = w / g.mask_w;
+        const int kb = (int)(w % g.mask_w) * 64;
+        const int i = (int)(row / g.yg), j = (int)(row % g.yg);
+        uint64_t bits = 0, gbits = 0;
+        for (int t = 0; t < 64; t++) {
+            const int k = kb + t;
+            if (k >= g.zg) break;
+            const uint64_t n = cell_index(g, i, j, k);
+            const T fx = F[3 * n], fy = F[3 * n + 1], fz = F[3 * n + 2];
+            const bool nz = fx != T(0) || fy != T(0) || fz != T(0) || signbit(fx) || signbit(fy) || signbit(fz);
+            if (nz) {
+                bits |= 1ull << t;
+                if (!is_interior(g, i, j, k)) gbits |= 1ull << t;
+            }
+        }
+        mask[w] = bits;
+        gmask[w] = gbits;
+    }
+}
+
+// F as the reference holds it: the stored value where the bit is set, else 0.
+template <typename T>
+__global__ void k_masked_f(const T* __restrict__ F, const uint64_t* __restrict__ mask, T* __restrict__ out, Grid g) {
+    for (uint64_t n = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; n < g.size;
+         n += (uint64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        decode_cell(g, n, i, j, k);
+        const bool on = mask_test(mask, g, i, j, k);
+        out[3 * n + 0] = on ? F[3 * n + 0] : T(0);
+        out[3 * n + 1] = on ? F[3 * n + 1] : T(0);
+        out[3 * n + 2] = on ? F[3 * n + 2] : T(0);
+    }
+}
+
+__global__ void k_zero_interior_o(uint8_t* __restrict__ O, Grid g) {
+    for (uint64_t n = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; n < g.size;
+         n += (uint64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        decode_cell(g, n, i, j, k);
+        if (is_interior(g, i, j, k)) O[n] = 0;
+    }
+}
+
+// ------------------------------------------------------------------ IB forcing
+__device__ __forceinline__ int wrap_ik(int a, int a1) {  // Forcing.hpp:85-88
+    if (a == 0) return a1;
+    if (a == a1 + 1) return 1;
+    return a;
+}
+
+// Before a forcing pass: F of a stencil cell that was not occupied after the
+// previous pass is zero in the reference (F=0 where O==0), so clear it here,
+// and mark the cell in the new occupancy mask.
+template <typename T>
+__global__ void k_force_prepare(PointsDev p, int64_t begin, int64_t count, T* __restrict__ F,
+                                const uint64_t* __restrict__ prev, uint64_t* __restrict__ next, Grid g) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int64_t q = begin + t;
+    const int pi = p.i[q], pj = p.j[q], pk = p.k[q];
+    for (int k1 = -1; k1 <= 1; k1++)
+        for (int i1 = -1; i1 <= 1; i1++) {
+            const int i2 = wrap_ik(pi + i1, g.xg1), k2 = wrap_ik(pk + k1, g.zg1);
+            const uint64_t c = cell_index(g, i2, pj, k2);
+            if (!mask_test(prev, g, i2, pj, k2)) {
+                F[3 * c + 0] = T(0);
+                F[3 * c + 1] = T(0);
+                F[3 * c + 2] = T(0);
+            }
+            atomicOr((unsigned long long*)&next[mask_word_index(g, i2, pj, k2)], 1ull << (k2 & 63));
+        }
+}
+
+// density and momentum of cell (i,j,k) as the reference's moments() would leave them
+template <typename T, int L, int STREAM, int MOMSRC, int EVEN>
+__device__ __forceinline__ void cell_rho_j(const T* __restrict__ Q, const Grid& g, int i, int j, int k, T& rho,
+                                           T& jx, T& jy, T& jz) {
+    const uint32_t n = cell_index(g, i, j, k);
+    if (STREAM == StreamEsotwist) {
+        const int32_t di = (i == g.xg1) ? -(int32_t)((uint32_t)(g.xg1 - 1) * g.yz) : (int32_t)g.yz;
+        const int32_t dj = (j == g.yg1) ? -(g.yg1 - 1) * g.zg : g.zg;
+        const int32_t dk = (k == g.zg1) ? -(g.zg1 - 1) : 1;
+        if (MOMSRC == MOMSRC_STORED) {
+            T m[4];
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                const uint32_t h = n + (uint32_t)((twist_dx(d) ? di : 0) + (twist_dy(d) ? dj : 0) + (twist_dz(d) ? dk : 0));
+                m[d] = Q[qaddr<L>(g, h, EVEN ? d : opposite(d))];
+            }
+            rho = m[0]; jx = m[1]; jy = m[2]; jz = m[3];
+        } else {
+            T f[kQ];
+#pragma unroll
+            for (int d = 0; d < kQ; d++) {
+                const uint32_t h = n + (uint32_t)((twist_dx(d) ? di : 0) + (twist_dy(d) ? dj : 0) + (twist_dz(d) ? dk : 0));
+                f[d] = Q[qaddr<L>(g, h, EVEN ? d : opposite(d))];
+            }
+            es_rho_j(f, rho, jx, jy, jz);
+        }
+    } else {
+        if (MOMSRC == MOMSRC_STORED) {
+            rho = Q[qaddr<L>(g, n, 0)];
+            jx = Q[qaddr<L>(g, n, 1)];
+            jy = Q[qaddr<L>(g, n, 2)];
+            jz = Q[qaddr<L>(g, n, 3)];
+        } else {
+            const bool im = i >= 2, ip = i <= g.xg1 - 1;
+            const bool jm = j >= 2, jp = j <= g.yg1 - 1;
+            const bool km = k >= 2, kp = k <= g.zg1 - 1;
+            T f[kQ];
+#pragma unroll
+            for (int d = 0; d < kQ; d++) {
+                const int ex = dir_x(d), ey = dir_y(d), ez = dir_z(d);
+                const bool ok = (ex > 0 ? im : ex < 0 ? ip : true) && (ey > 0 ? jm : ey < 0 ? jp : true) &&
+                                (ez > 0 ? km : ez < 0 ? kp : true);
+                const int32_t off = -(ex * (int32_t)g.yz + ey * g.zg + ez);
+                f[d] = Q[qaddr<L>(g, ok ? n + (uint32_t)off : n, d)];
+            }
+            es_rho_j(f, rho, jx, jy, jz);
+        }
+    }
+}
+
+// One level of the forcing: points of a level have pairwise disjoint stencils,
+// so each thread runs the reference's gather / correct / spread for one point
+// (Forcing.hpp:73-135) with the F values left by all earlier levels.
+template <typename T, int L, int STREAM, int MOMSRC, int EVEN>
+__global__ __launch_bounds__(kBlock) void k_force_level(PointsDev p, int64_t begin, int64_t count,
+                                                        T* __restrict__ F, const T* __restrict__ Q, Grid g,
+                                                        T alfa, T beta) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int64_t q = begin + t;
+    const int pi = p.i[q], pj = p.j[q], pk = p.k[q];
+    const T ifr = static_cast<const T*>(p.ifrac)[q], kfr = static_cast<const T*>(p.kfrac)[q];
+    T wx[3], wz[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        wx[a] = ib_weight<T>(-ifr + T(a - 1));
+        wz[a] = ib_weight<T>(-kfr + T(a - 1));
+    }
+    T rhoSum = T(0.0), xSum = T(0.0), ySum = T(0.0), zSum = T(0.0);
+#pragma unroll
+    for (int k1 = -1; k1 <= 1; k1++)
+#pragma unroll
+        for (int i1 = -1; i1 <= 1; i1++) {
+            const int i2 = wrap_ik(pi + i1, g.xg1), k2 = wrap_ik(pk + k1, g.zg1);
+            T rho, jx, jy, jz;
+            cell_rho_j<T, L, STREAM, MOMSRC, EVEN>(Q, g, i2, pj, k2, rho, jx, jy, jz);
+            const uint64_t c = cell_index(g, i2, pj, k2);
+            const T ppp = wx[i1 + 1] * wz[k1 + 1];
+            const T xx = (T)((double)jx + 0.5 * (double)F[3 * c + 0]);
+            const T yy = (T)((double)jy + 0.5 * (double)F[3 * c + 1]);
+            const T zz = (T)((double)jz + 0.5 * (double)F[3 * c + 2]);
+            rhoSum += ppp * rho;
+            xSum += ppp * xx;
+            ySum += ppp * yy;
+            zSum += ppp * zz;
+        }
+    xSum -= rhoSum * static_cast<const T*>(p.u)[q];
+    ySum -= rhoSum * static_cast<const T*>(p.v)[q];
+    zSum -= rhoSum * static_cast<const T*>(p.w)[q];
+#pragma unroll
+    for (int k1 = -1; k1 <= 1; k1++)
+#pragma unroll
+        for (int i1 = -1; i1 <= 1; i1++) {
+            const int i2 = wrap_ik(pi + i1, g.xg1), k2 = wrap_ik(pk + k1, g.zg1);
+            const uint64_t c = cell_index(g, i2, pj, k2);
+            const T bp = beta * (wx[i1 + 1] * wz[k1 + 1]);
+            F[3 * c + 0] = alfa * F[3 * c + 0] - bp * xSum;
+            F[3 * c + 1] = alfa * F[3 * c + 1] - bp * ySum;
+            F[3 * c + 2] = alfa * F[3 * c + 2] - bp * zSum;
+        }
+}
+
+// ------------------------------------------------------------------ geometry on the device
+template <typename T>
+__global__ void k_gen_blades(BladeLayout bl, BladeStep st, PointsDev out, int64_t offset, int64_t count) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    BladeEnum e;
+    e.n_blades = bl.n_blades;
+    e.n_planes = bl.n_planes;
+    e.n_radial = bl.n_radial;
+    e.n_bcol = (bl.n_blades % 2 == 0) ? 2 : 3;
+    int b, pl, r, cls;
+    blade_decode(e, t, b, pl, r, cls);
+    tdlb_ib_point pt;
+    blade_point(bl.cx, bl.cz, bl.r_inner, bl.j_lo + pl, r, st.cosb[b], st.sinb[b], st.omega, &pt);
+    const int64_t q = offset + t;
+    out.i[q] = (int32_t)(pt.i + bl.ghost);
+    out.j[q] = (int32_t)(pt.j + bl.ghost);
+    out.k[q] = (int32_t)(pt.k + bl.ghost);
+    static_cast<T*>(out.ifrac)[q] = (T)pt.i_frac;
+    static_cast<T*>(out.kfrac)[q] = (T)pt.k_frac;
+    static_cast<T*>(out.u)[q] = (T)pt.u_delta;
+    static_cast<T*>(out.v)[q] = (T)pt.v_delta;
+    static_cast<T*>(out.w)[q] = (T)pt.w_delta;
+}
+
+// getExternalPoints as a mask: a cell is outside when its centre lies beyond the
+// tank radius from the axis (exact: integer and half-integer coordinates).
+__global__ void k_solid_mask(double cx, double cz, double radius, uint8_t* __restrict__ out, Grid g, int ghost) {
+    for (uint64_t n = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; n < g.size;
+         n += (uint64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        decode_cell(g, n, i, j, k);
+        const double dx = (double)(i - ghost) - cx, dz = (double)(k - ghost) - cz;
+        out[n] = (dx * dx + dz * dz > radius * radius) ? 1 : 0;
+    }
+}
+
+template <typename T>
+CollideParams<T> to_params(const CollideArgs& c) {
+    return make_collide_params<T>(T(c.nu), T(c.cs0), T(c.g3), c.use_les);
+}
+
+uint32_t grid_blocks(uint64_t n) {
+    uint64_t b = (n + kBlock - 1) / kBlock;
+    if (b > 65536ull * 16) b = 65536ull * 16;
+    return (uint32_t)(b ? b : 1);
+}
+
+// rows of interior cells: blocks of kBlock threads along k
+inline void row_launch(const Grid& g, uint32_t& nkb, uint32_t& nblocks) {
+    nkb = (uint32_t)((g.zg1 + kBlock - 1) / kBlock);
+    nblocks = nkb * (uint32_t)g.yg1 * (uint32_t)g.xg1;
+}
+
+template <typename T, int L>
+hipError_t esotwist_t(int mode, int even, void* Q, const void* F, void* Nu, const uint64_t* fm, const Grid& g,
+                      const CollideParams<T>& P, hipStream_t s) {
+    uint32_t nkb, nb;
+    row_launch(g, nkb, nb);
+    T* q = static_cast<T*>(Q);
+    const T* f = static_cast<const T*>(F);
+    T* nu = static_cast<T*>(Nu);
+#define TDLB_ESO(M, E) hipLaunchKernelGGL((k_esotwist<T, L, M, E>), dim3(nb), dim3(kBlock), 0, s, q, f, nu, fm, g, P, nkb)
+    if (mode == MODE_COLL) { if (even) TDLB_ESO(MODE_COLL, 1); else TDLB_ESO(MODE_COLL, 0); }
+    else if (mode == MODE_MOM) { if (even) TDLB_ESO(MODE_MOM, 1); else TDLB_ESO(MODE_MOM, 0); }
+    else if (mode == MODE_FUSED) { if (even) TDLB_ESO(MODE_FUSED, 1); else TDLB_ESO(MODE_FUSED, 0); }
+    else return hipErrorInvalidValue;
+#undef TDLB_ESO
+    return hipGetLastError();
+}
+
+template <typename T, int L>
+hipError_t simple_t(int mode, const void* src, void* dst, const void* F, void* Nu, const uint64_t* fm,
+                    const Grid& g, const CollideParams<T>& P, hipStream_t s) {
+    uint32_t nkb, nb;
+    row_launch(g, nkb, nb);
+    const T* a = static_cast<const T*>(src);
+    T* b = static_cast<T*>(dst);
+    const T* f = static_cast<const T*>(F);
+    T* nu = static_cast<T*>(Nu);
+#define TDLB_SIM(M) hipLaunchKernelGGL((k_simple<T, L, M>), dim3(nb), dim3(kBlock), 0, s, a, b, f, nu, fm, g, P, nkb)
+    switch (mode) {
+        case MODE_COLL: TDLB_SIM(MODE_COLL); break;
+        case MODE_MOM: TDLB_SIM(MODE_MOM); break;
+        case MODE_FUSED: TDLB_SIM(MODE_FUSED); break;
+        case MODE_MOM_PULL: TDLB_SIM(MODE_MOM_PULL); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef TDLB_SIM
+    return hipGetLastError();
+}
+
+template <typename T, int L, int STREAM>
+hipError_t force_level_t(int momsrc, int even, const PointsDev& p, int64_t begin, int64_t count, void* F,
+                         const void* Q, const Grid& g, T alfa, T beta, hipStream_t s) {
+    const uint32_t nb = (uint32_t)((count + kBlock - 1) / kBlock);
+    T* f = static_cast<T*>(F);
+    const T* q = static_cast<const T*>(Q);
+#define TDLB_FL(M, E) \
+    hipLaunchKernelGGL((k_force_level<T, L, STREAM, M, E>), dim3(nb), dim3(kBlock), 0, s, p, begin, count, f, q, g, alfa, beta)
+    if (momsrc == MOMSRC_STORED) { if (even) TDLB_FL(MOMSRC_STORED, 1); else TDLB_FL(MOMSRC_STORED, 0); }
+    else if (momsrc == MOMSRC_ESO_POPS) { if (even) TDLB_FL(MOMSRC_ESO_POPS, 1); else TDLB_FL(MOMSRC_ESO_POPS, 0); }
+    else { TDLB_FL(MOMSRC_PULL_POPS, 1); }
+#undef TDLB_FL
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// ====================================================================== launchers
+hipError_t launch_fill_q(int prec, int layout, void* Q, const Grid& g, double rho, hipStream_t s) {
+    const uint32_t nb = grid_blocks(g.size * kQ);
+    if (prec == 0) {
+        if (layout == LayoutIJKL) hipLaunchKernelGGL((k_fill_q<float, LayoutIJKL>), dim3(nb), dim3(kBlock), 0, s, (float*)Q, g, (float)rho);
+        else hipLaunchKernelGGL((k_fill_q<float, LayoutLIJK>), dim3(nb), dim3(kBlock), 0, s, (float*)Q, g, (float)rho);
+    } else {
+        if (layout == LayoutIJKL) hipLaunchKernelGGL((k_fill_q<double, LayoutIJKL>), dim3(nb), dim3(kBlock), 0, s, (double*)Q, g, rho);
+        else hipLaunchKernelGGL((k_fill_q<double, LayoutLIJK>), dim3(nb), dim3(kBlock), 0, s, (double*)Q, g, rho);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    if (bytes % 4 == 0 && ((uintptr_t)p % 4) == 0)
+        hipLaunchKernelGGL(k_zero_words, dim3(grid_blocks(bytes / 4)), dim3(kBlock), 0, s, (uint32_t*)p, (uint64_t)(bytes / 4));
+    else
+        hipLaunchKernelGGL(k_zero_bytes, dim3(grid_blocks(bytes)), dim3(kBlock), 0, s, (uint8_t*)p, (uint64_t)bytes);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    if (bytes % 4 == 0 && ((uintptr_t)dst % 4) == 0 && ((uintptr_t)src % 4) == 0)
+        hipLaunchKernelGGL(k_copy_words, dim3(grid_blocks(bytes / 4)), dim3(kBlock), 0, s, (uint32_t*)dst,
+                           (const uint32_t*)src, (uint64_t)(bytes / 4));
+    else
+        hipLaunchKernelGGL(k_copy_bytes, dim3(grid_blocks(bytes)), dim3(kBlock), 0, s, (uint8_t*)dst,
+                           (const uint8_t*)src, (uint64_t)bytes);
+    return hipGetLastError();
+}
+
+hipError_t launch_esotwist(int prec, int layout, int mode, int even, void* Q, const void* F, void* Nu,
+                           const uint64_t* fmask, const Grid& g, const CollideArgs& c, hipStream_t s) {
+    if (prec == 0) {
+        const auto P = to_params<float>(c);
+        return layout == LayoutIJKL ? esotwist_t<float, LayoutIJKL>(mode, even, Q, F, Nu, fmask, g, P, s)
+                                    : esotwist_t<float, LayoutLIJK>(mode, even, Q, F, Nu, fmask, g, P, s);
+    }
+    const auto P = to_params<double>(c);
+    return layout == LayoutIJKL ? esotwist_t<double, LayoutIJKL>(mode, even, Q, F, Nu, fmask, g, P, s)
+                                : esotwist_t<double, LayoutLIJK>(mode, even, Q, F, Nu, fmask, g, P, s);
+}
+
+hipError_t launch_simple(int prec, int layout, int mode, const void* src, void* dst, const void* F, void* Nu,
+                         const uint64_t* fmask, const Grid& g, const CollideArgs& c, hipStream_t s) {
+    if (prec == 0) {
+        const auto P = to_params<float>(c);
+        return layout == LayoutIJKL ? simple_t<float, LayoutIJKL>(mode, src, dst, F, Nu, fmask, g, P, s)
+                                    : simple_t<float, LayoutLIJK>(mode, src, dst, F, Nu, fmask, g, P, s);
+    }
+    const auto P = to_params<double>(c);
+    return layout == LayoutIJKL ? simple_t<double, LayoutIJKL>(mode, src, dst, F, Nu, fmask, g, P, s)
+                                : simple_t<double, LayoutLIJK>(mode, src, dst, F, Nu, fmask, g, P, s);
+}
+
+// launch KERNEL<W, layout> over NTHREADS grid-stride threads
+#define TDLB_MOVE(KERNEL, W, NTHREADS, ...)                                                                     \
+    do {                                                                                                        \
+        const uint32_t nb_ = grid_blocks(NTHREADS);                                                             \
+        if (layout == LayoutIJKL)                                                                               \
+            hipLaunchKernelGGL((KERNEL<W, LayoutIJKL>), dim3(nb_), dim3(kBlock), 0, s, __VA_ARGS__);            \
+        else                                                                                                    \
+            hipLaunchKernelGGL((KERNEL<W, LayoutLIJK>), dim3(nb_), dim3(kBlock), 0, s, __VA_ARGS__);            \
+    } while (0)
+
+hipError_t launch_push_all(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s) {
+    if (esize == 4) TDLB_MOVE(k_push_all, uint32_t, g.size, (const uint32_t*)src, (uint32_t*)dst, g);
+    else TDLB_MOVE(k_push_all, uint64_t, g.size, (const uint64_t*)src, (uint64_t*)dst, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_pull_all(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s) {
+    if (esize == 4) TDLB_MOVE(k_pull_all, uint32_t, g.size, (const uint32_t*)src, (uint32_t*)dst, g);
+    else TDLB_MOVE(k_pull_all, uint64_t, g.size, (const uint64_t*)src, (uint64_t*)dst, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_bounce_back(int esize, int layout, void* Q, const Grid& g, hipStream_t s) {
+    for (int face = 0; face < 6; face++) {
+        const int axis = face >> 1;
+        const uint64_t n = (uint64_t)(axis == 0 ? g.yg1 : g.xg1) * (axis == 2 ? g.yg1 : g.zg1);
+        if (esize == 4) TDLB_MOVE(k_bb_face, uint32_t, n, (uint32_t*)Q, g, face);
+        else TDLB_MOVE(k_bb_face, uint64_t, n, (uint64_t*)Q, g, face);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    const uint64_t n = 12ull * (uint64_t)std::max(g.xg1, std::max(g.yg1, g.zg1));
+    if (esize == 4) TDLB_MOVE(k_bb_edges, uint32_t, n, (uint32_t*)Q, g);
+    else TDLB_MOVE(k_bb_edges, uint64_t, n, (uint64_t*)Q, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_ghost_fill(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s) {
+    if (esize == 4) TDLB_MOVE(k_ghost_fill, uint32_t, ghost_count(g), (const uint32_t*)src, (uint32_t*)dst, g);
+    else TDLB_MOVE(k_ghost_fill, uint64_t, ghost_count(g), (const uint64_t*)src, (uint64_t*)dst, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_ghost_copy(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s) {
+    if (esize == 4) TDLB_MOVE(k_ghost_copy, uint32_t, ghost_count(g), (const uint32_t*)src, (uint32_t*)dst, g);
+    else TDLB_MOVE(k_ghost_copy, uint64_t, ghost_count(g), (const uint64_t*)src, (uint64_t*)dst, g);
+    return hipGetLastError();
+}
+#undef TDLB_MOVE
+
+hipError_t launch_mask_from_f(int prec, const void* F, uint64_t* mask, uint64_t* gmask, const Grid& g,
+                              hipStream_t s) {
+    const uint64_t words = (uint64_t)g.xg * g.yg * g.mask_w;
+    if (prec == 0) hipLaunchKernelGGL(k_mask_from_f<float>, dim3(grid_blocks(words)), dim3(kBlock), 0, s, (const float*)F, mask, gmask, g);
+    else hipLaunchKernelGGL(k_mask_from_f<double>, dim3(grid_blocks(words)), dim3(kBlock), 0, s, (const double*)F, mask, gmask, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_masked_f(int prec, const void* F, const uint64_t* mask, void* out, const Grid& g, hipStream_t s) {
+    if (prec == 0) hipLaunchKernelGGL(k_masked_f<float>, dim3(grid_blocks(g.size)), dim3(kBlock), 0, s, (const float*)F, mask, (float*)out, g);
+    else hipLaunchKernelGGL(k_masked_f<double>, dim3(grid_blocks(g.size)), dim3(kBlock), 0, s, (const double*)F, mask, (double*)out, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_interior_o(uint8_t* O, const Grid& g, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_interior_o, dim3(grid_blocks(g.size)), dim3(kBlock), 0, s, O, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_force_prepare(int prec, const PointsDev& p, int64_t begin, int64_t count, void* F,
+                                const uint64_t* prev, uint64_t* next, const Grid& g, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    const uint32_t nb = (uint32_t)((count + kBlock - 1) / kBlock);
+    if (prec == 0) hipLaunchKernelGGL(k_force_prepare<float>, dim3(nb), dim3(kBlock), 0, s, p, begin, count, (float*)F, prev, next, g);
+    else hipLaunchKernelGGL(k_force_prepare<double>, dim3(nb), dim3(kBlock), 0, s, p, begin, count, (double*)F, prev, next, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_force_level(int prec, int layout, int streaming, int momsrc, int even, const PointsDev& p,
+                              int64_t begin, int64_t count, void* F, const void* Q, const Grid& g, double alfa,
+                              double beta, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    if (prec == 0) {
+        const float a = (float)alfa, b = (float)beta;
+        if (layout == LayoutIJKL) {
+            return streaming == StreamEsotwist
+                       ? force_level_t<float, LayoutIJKL, StreamEsotwist>(momsrc, even, p, begin, count, F, Q, g, a, b, s)
+                       : force_level_t<float, LayoutIJKL, StreamSimple>(momsrc, even, p, begin, count, F, Q, g, a, b, s);
+        }
+        return streaming == StreamEsotwist
+                   ? force_level_t<float, LayoutLIJK, StreamEsotwist>(momsrc, even, p, begin, count, F, Q, g, a, b, s)
+                   : force_level_t<float, LayoutLIJK, StreamSimple>(momsrc, even, p, begin, count, F, Q, g, a, b, s);
+    }
+    if (layout == LayoutIJKL) {
+        return streaming == StreamEsotwist
+                   ? force_level_t<double, LayoutIJKL, StreamEsotwist>(momsrc, even, p, begin, count, F, Q, g, alfa, beta, s)
+                   : force_level_t<double, LayoutIJKL, StreamSimple>(momsrc, even, p, begin, count, F, Q, g, alfa, beta, s);
+    }
+    return streaming == StreamEsotwist
+               ? force_level_t<double, LayoutLIJK, StreamEsotwist>(momsrc, even, p, begin, count, F, Q, g, alfa, beta, s)
+               : force_level_t<double, LayoutLIJK, StreamSimple>(momsrc, even, p, begin, count, F, Q, g, alfa, beta, s);
+}
+
+hipError_t launch_gen_blades(int prec, const BladeLayout& bl, const BladeStep& st, const PointsDev& out,
+                             int64_t offset, hipStream_t s) {
+    const int64_t count = (int64_t)bl.n_blades * bl.n_planes * bl.n_radial;
+    if (count <= 0) return hipSuccess;
+    const uint32_t nb = (uint32_t)((count + kBlock - 1) / kBlock);
+    if (prec == 0) hipLaunchKernelGGL(k_gen_blades<float>, dim3(nb), dim3(kBlock), 0, s, bl, st, out, offset, count);
+    else hipLaunchKernelGGL(k_gen_blades<double>, dim3(nb), dim3(kBlock), 0, s, bl, st, out, offset, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_solid_mask(double cx, double cz, double radius, uint8_t* out, const Grid& g, int ghost,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_solid_mask, dim3(grid_blocks(g.size)), dim3(kBlock), 0, s, cx, cz, radius, out, g, ghost);
+    return hipGetLastError();
+}
+
+}  // namespace tdlb

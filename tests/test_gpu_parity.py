@@ -1,0 +1,228 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle on the same
+seeded inputs, and with the reference's golden streaming vectors.
+
+Bar: bit-exact for data movement, index lists and masks; for the float path
+the reference bar is 1e-5 relative after 20 steps (|a-b| <= 1e-5 max(|b|, rho0/24)),
+and the tests also record whether the result is bit-identical.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import tdlbcpp_amd as td
+from oracle import oracle as O
+from tests.helpers import ReferenceLoop, assert_close_moments, flow_for_tank, random_moments
+
+pytestmark = pytest.mark.gpu
+
+LAYOUTS = [(td.LAYOUT_IJKL, O.IJKL), (td.LAYOUT_LIJK, O.LIJK)]
+STREAMS = [(td.STREAM_SIMPLE, O.SIMPLE), (td.STREAM_ESOTWIST, O.ESOTWIST)]
+
+
+def case(msg):
+    print(f"[gpu case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def tag_pair(layout, x=3, y=3, z=3):
+    ocu = O.OracleCU(x, y, z, dtype=O.TAGS, layout=layout[1], streaming=O.SIMPLE)
+    q, f, nu = O.fill_for_test(ocu)
+    ocu.set_q_ijkl(q)
+    gcu = td.ComputeUnit(x, y, z, layout=layout[0], streaming=td.STREAM_SIMPLE, precision=td.PREC_F64)
+    gcu.set(td.FIELD_Q, ocu.get(O.FIELD_Q))
+    return ocu, gcu
+
+
+# ---------------------------------------------------------------- data movement
+@pytest.mark.parametrize("layout", LAYOUTS, ids=["IJKL", "LIJK"])
+def test_gpu_push_matches_reference_fixture(layout, golden_dir):
+    case(f"push golden {layout}")
+    _, g = tag_pair(layout)
+    g.streaming_push()
+    got = g.q_ijkl(dtype=np.uint64)
+    np.testing.assert_array_equal(got, np.load(f"{golden_dir}/streaming_push_Q.npy"))
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=["IJKL", "LIJK"])
+def test_gpu_pull_matches_reference_fixture(layout, golden_dir):
+    case(f"pull golden {layout}")
+    _, g = tag_pair(layout)
+    g.streaming_pull()
+    np.testing.assert_array_equal(g.q_ijkl(dtype=np.uint64), np.load(f"{golden_dir}/streaming_pull_Q.npy"))
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=["IJKL", "LIJK"])
+@pytest.mark.parametrize("op", ["push", "pull", "bounce_back", "bb_then_push"])
+def test_gpu_data_movement_matches_oracle_nonsquare(layout, op):
+    case(f"data movement {op} {layout}")
+    o, g = tag_pair(layout, 4, 3, 5)
+    steps = {"push": ["streaming_push"], "pull": ["streaming_pull"], "bounce_back": ["bounce_back"],
+             "bb_then_push": ["bounce_back", "streaming_push", "bounce_back", "streaming_push"]}[op]
+    for s in steps:
+        getattr(o, s)()
+        getattr(g, s)()
+    np.testing.assert_array_equal(g.get(td.FIELD_Q, np.uint64), o.get(O.FIELD_Q))
+
+
+# ---------------------------------------------------------------- collision / moments
+def seeded_pair(layout, stream, prec, les, shape=(7, 6, 9), seed=1, even=True):
+    rng = np.random.default_rng(seed)
+    odt = O.F32 if prec == td.PREC_F32 else O.F64
+    nu = 0.003
+    o = O.OracleCU(*shape, dtype=odt, layout=layout[1], streaming=stream[1], use_les=les, nu=nu, cs0=0.12,
+                   g3=0.8)
+    g = td.ComputeUnit(*shape, layout=layout[0], streaming=stream[0], precision=prec, nu=nu, cs0=0.12, g3=0.8,
+                       use_les=1 if les else 0)
+    xg, yg, zg = o.xg, o.yg, o.zg
+    m = random_moments((xg, yg, zg), rng)
+    o.set_q_ijkl(m)
+    F = np.zeros((xg, yg, zg, 3))
+    sel = rng.random((xg, yg, zg)) < 0.2
+    F[sel] = 1e-4 * rng.standard_normal((sel.sum(), 3))
+    o.set(O.FIELD_F, F.reshape(-1))
+    nuv = 1e-4 * rng.random((xg, yg, zg))
+    o.set(O.FIELD_NU, nuv.reshape(-1))
+    for f in (O.FIELD_Q, O.FIELD_F, O.FIELD_NU):
+        g.set({O.FIELD_Q: td.FIELD_Q, O.FIELD_F: td.FIELD_F, O.FIELD_NU: td.FIELD_NU}[f], o.get(f))
+    o.even_step = even
+    g.even_step = even
+    return o, g
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=["IJKL", "LIJK"])
+@pytest.mark.parametrize("stream", STREAMS, ids=["Simple", "Esotwist"])
+@pytest.mark.parametrize("prec", [td.PREC_F32, td.PREC_F64], ids=["f32", "f64"])
+@pytest.mark.parametrize("les", [False, True], ids=["noLES", "LES"])
+def test_gpu_collision_and_moments_bit_exact(layout, stream, prec, les):
+    for even in (True, False) if stream[0] == td.STREAM_ESOTWIST else (True,):
+        case(f"collision {layout} {stream} prec={prec} les={les} even={even}")
+        o, g = seeded_pair(layout, stream, prec, les, even=even)
+        o.collision()
+        g.collision()
+        np.testing.assert_array_equal(g.get(td.FIELD_Q), o.get(O.FIELD_Q))
+        np.testing.assert_array_equal(g.get(td.FIELD_NU), o.get(O.FIELD_NU))
+        o.moments()
+        g.moments()
+        np.testing.assert_array_equal(g.get(td.FIELD_Q), o.get(O.FIELD_Q))
+
+
+# ---------------------------------------------------------------- forcing
+def random_points(rng, shape, n):
+    pts = np.zeros(n, dtype=td.POINT_DTYPE)
+    pts["i"] = rng.integers(0, shape[0], n)
+    pts["j"] = rng.integers(0, shape[1], n)
+    pts["k"] = rng.integers(0, shape[2], n)
+    pts["i_frac"] = rng.uniform(-0.5, 0.5, n)
+    pts["k_frac"] = rng.uniform(-0.5, 0.5, n)
+    pts["u_delta"] = rng.uniform(-0.05, 0.05, n)
+    pts["v_delta"] = rng.uniform(-0.05, 0.05, n)
+    pts["w_delta"] = rng.uniform(-0.05, 0.05, n)
+    return pts
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=["IJKL", "LIJK"])
+@pytest.mark.parametrize("stream", STREAMS, ids=["Simple", "Esotwist"])
+def test_gpu_forcing_bit_exact_with_overlaps_and_wrap(layout, stream):
+    case(f"forcing {layout} {stream}")
+    shape = (8, 5, 7)
+    o, g = seeded_pair(layout, stream, td.PREC_F32, False, shape=shape, seed=3)
+    rng = np.random.default_rng(4)
+    # dense overlapping list incl. points on the first/last interior rows (periodic wrap)
+    pts = random_points(rng, shape, 120)
+    for call in range(3):
+        o.forcing(pts, 0.97, 1.9)
+        g.forcing(pts, 0.97, 1.9)
+        np.testing.assert_array_equal(g.get(td.FIELD_F), o.get(O.FIELD_F))
+        np.testing.assert_array_equal(g.get(td.FIELD_O), o.get(O.FIELD_O))
+        pts = random_points(rng, shape, 40 + 30 * call)
+    g.forcing(pts[:0], 0.97, 1.9)
+    o.forcing(pts[:0], 0.97, 1.9)
+    np.testing.assert_array_equal(g.get(td.FIELD_F), o.get(O.FIELD_F))
+
+
+# ---------------------------------------------------------------- main loop
+@pytest.mark.parametrize("layout", LAYOUTS, ids=["IJKL", "LIJK"])
+@pytest.mark.parametrize("stream", STREAMS, ids=["Simple", "Esotwist"])
+def test_gpu_main_loop_20_steps_matches_oracle(layout, stream):
+    T = 32
+    case(f"main loop T={T} 20 steps {layout} {stream}")
+    oflow, gflow = flow_for_tank(T)
+    o = O.OracleCU(T, dtype=O.F32, layout=layout[1], streaming=stream[1], use_les=True, **oflow)
+    g = td.ComputeUnit(T, layout=layout[0], streaming=stream[0], precision=td.PREC_F32, **gflow)
+    rp = td.rushton_defaults(T)
+    o.initialise(1.0)
+    g.initialise(1.0)
+    ref = ReferenceLoop(o, rp)
+    g.bind_rushton(rp, step=1, angle=0.0)
+    np.testing.assert_array_equal(g.get(td.FIELD_F), o.get(O.FIELD_F))
+    ref.run(20)
+    g.run(20)
+    assert g.step == ref.step and g.angle == ref.angle
+    assert_close_moments(g.get(td.FIELD_Q), o.get(O.FIELD_Q), what="Q")
+    assert_close_moments(g.get(td.FIELD_F), o.get(O.FIELD_F), what="F", rho0=1e-3)
+    assert_close_moments(g.get(td.FIELD_NU), o.get(O.FIELD_NU), what="Nu", rho0=1e-3)
+    exact = np.array_equal(g.get(td.FIELD_Q), o.get(O.FIELD_Q))
+    print(f"[gpu case] bit-identical Q after 20 steps: {exact}", flush=True)
+    assert exact
+
+
+@pytest.mark.parametrize("stream", STREAMS, ids=["Simple", "Esotwist"])
+def test_gpu_eager_calls_equal_fused_run(stream):
+    """The five reference calls per step and the fused fast path leave identical state."""
+    T = 30
+    case(f"eager vs fused T={T} {stream}")
+    _, gflow = flow_for_tank(T)
+    rp = td.rushton_defaults(T)
+    a = td.ComputeUnit(T, layout=td.LAYOUT_LIJK, streaming=stream[0], **gflow)
+    b = td.ComputeUnit(T, layout=td.LAYOUT_LIJK, streaming=stream[0], **gflow)
+    a.initialise(1.0)
+    b.initialise(1.0)
+    a.bind_rushton(rp)
+    fixed = td.rushton_generate(rp, td.GEOM_FIXED)
+    d0 = float(np.float32(td.rushton_step_increment(rp, 1)))
+    nonupd = td.rushton_generate(rp, td.GEOM_ROTATING_NON_UPDATING, 0.0, d0)
+    b.forcing(np.concatenate([fixed, nonupd, td.rushton_generate(rp, td.GEOM_ROTATING, 0.0, d0)]), 0.97, 1.9)
+    step, angle = 1, 0.0
+    for _ in range(6):
+        step += 1
+        d = float(np.float32(td.rushton_step_increment(rp, step)))
+        angle += d
+        b.collision()
+        b.bounce_back()
+        b.streaming_push()
+        b.moments()
+        b.forcing(np.concatenate([fixed, nonupd, td.rushton_generate(rp, td.GEOM_ROTATING, angle, d)]), 0.97, 1.9)
+    a.run(6)
+    for f in (td.FIELD_Q, td.FIELD_F, td.FIELD_NU, td.FIELD_O):
+        np.testing.assert_array_equal(a.get(f), b.get(f))
+    assert a.even_step == b.even_step
+
+
+def test_gpu_rotating_points_bit_exact_with_host_generator():
+    T = 60
+    case(f"blade list T={T}")
+    _, gflow = flow_for_tank(T)
+    rp = td.rushton_defaults(T)
+    g = td.ComputeUnit(T, layout=td.LAYOUT_LIJK, streaming=td.STREAM_ESOTWIST, **gflow)
+    g.initialise(1.0)
+    g.bind_rushton(rp)
+    for n in (1, 7, 31):
+        g.run(n)
+        d = float(np.float32(td.rushton_step_increment(rp, g.step)))
+        host = td.rushton_generate(rp, td.GEOM_ROTATING, g.angle, d)
+        dev = g.rotating_points()
+        for name in ("i", "j", "k"):
+            np.testing.assert_array_equal(dev[name], host[name])
+        for name in ("i_frac", "k_frac", "u_delta", "v_delta", "w_delta"):
+            np.testing.assert_array_equal(dev[name], host[name].astype(np.float32).astype(np.float64))
+
+
+def test_gpu_solid_mask_bit_exact():
+    T = 48
+    case(f"solid mask T={T}")
+    rp = td.rushton_defaults(T)
+    g = td.ComputeUnit(T, layout=td.LAYOUT_LIJK, streaming=td.STREAM_ESOTWIST)
+    g.initialise(1.0)
+    g.bind_rushton(rp)
+    np.testing.assert_array_equal(g.solid_mask(), td.rushton_solid_mask(rp, g.xg, g.yg, g.zg, 1))
