@@ -29,7 +29,9 @@
 // every step).  Static groups are emitted in colour order (i mod 3, k mod 3) of
 // their stencil centre so that few points overlap within a run; blades are
 // emitted colour-major (blade parity, radial index mod 5), which makes points of
-// one colour pairwise non-overlapping for T >= 30.
+// one colour pairwise non-overlapping whenever the blades of one colour are
+// at least 3*sqrt(2) cells apart at the inner radius (checked at bind time; true
+// for T = 30 and every T >= 36).
 #pragma once
 
 #include <cstdint>

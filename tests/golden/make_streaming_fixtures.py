@@ -13,6 +13,8 @@ with integer tags ((i*100+j)*100+k)*100+l before streaming.
 
 Output: tests/golden/streaming_{pull,push}_Q.npy  uint64 [5,5,5,18] (i,j,k,slot)
         tests/golden/streaming_push_F.npy         uint64 [5,5,5,3]
+        tests/golden/input_debug_gridx60_numSteps20.json        the reference's sample
+        tests/golden/input_debug_gridx72_IncreaseResolution.json input files (data)
 """
 import os
 import re
@@ -63,6 +65,15 @@ def main():
     np.save(os.path.join(HERE, "streaming_push_Q.npy"), push_q)
     np.save(os.path.join(HERE, "streaming_push_F.npy"), push_f)
     print("wrote", pull_q.shape, push_q.shape, push_f.shape)
+    # sample input files: JSON data, copied after a parse check
+    import json
+    for name in ("input_debug_gridx60_numSteps20.json", "input_debug_gridx72_IncreaseResolution.json"):
+        with open(os.path.join(REF, name)) as fh:
+            text = fh.read()
+        json.loads(text)
+        with open(os.path.join(HERE, name), "w") as fh:
+            fh.write(text)
+        print("wrote", name)
     return 0
 
 

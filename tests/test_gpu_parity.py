@@ -145,7 +145,7 @@ def test_gpu_forcing_bit_exact_with_overlaps_and_wrap(layout, stream):
 @pytest.mark.parametrize("layout", LAYOUTS, ids=["IJKL", "LIJK"])
 @pytest.mark.parametrize("stream", STREAMS, ids=["Simple", "Esotwist"])
 def test_gpu_main_loop_20_steps_matches_oracle(layout, stream):
-    T = 32
+    T = 36  # smallest tank above 30 whose blades pass the on-device colouring check
     case(f"main loop T={T} 20 steps {layout} {stream}")
     oflow, gflow = flow_for_tank(T)
     o = O.OracleCU(T, dtype=O.F32, layout=layout[1], streaming=stream[1], use_les=True, **oflow)

@@ -104,7 +104,7 @@ def test_rushton_blades_rotate_rigidly_and_have_sixfold_symmetry():
 def test_blade_colour_classes_are_conflict_free():
     """Blades are emitted colour-major; inside a colour no two 3x3 (i,k) stencils on
     one plane share a cell, so each colour is one parallel level of the forcing
-    (T >= 30)."""
+    (T = 30 and T >= 36)."""
     for T in (30, 60, 128):
         p = td.rushton_defaults(T)
         n_radial = int(math.floor(p.blade_outer_radius - p.blade_inner_radius + 1e-9)) + 1
