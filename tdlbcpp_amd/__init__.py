@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libtdlb_hip.so")
 
 LAYOUT_IJKL, LAYOUT_LIJK = 0, 1
 STREAM_SIMPLE, STREAM_ESOTWIST = 0, 1
-PREC_F32, PREC_F64 = 0, 1
+PREC_F32, PREC_F64, PREC_F16, PREC_BF16 = 0, 1, 2, 3
 FIELD_Q, FIELD_F, FIELD_NU, FIELD_O = 0, 1, 2, 3
 GEOM_FIXED, GEOM_ROTATING_NON_UPDATING, GEOM_ROTATING = 0, 1, 2
 
@@ -200,7 +200,9 @@ class ComputeUnit:
         check(lib().tdlb_cu_dims(self._h, ctypes.byref(xg), ctypes.byref(yg), ctypes.byref(zg), ctypes.byref(size)))
         self.xg, self.yg, self.zg, self.size = xg.value, yg.value, zg.value, size.value
         self.layout = layout
-        self.np_dtype = np.float32 if precision == PREC_F32 else np.float64
+        self.precision = precision
+        # every field is float except with fp64 (the 16-bit modes store only the lattice in 16 bits)
+        self.np_dtype = np.float64 if precision == PREC_F64 else np.float32
         self.step = 0
         self.angle = 0.0
 

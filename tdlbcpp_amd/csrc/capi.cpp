@@ -17,6 +17,12 @@
 // F occupancy: fmask[fm] has a bit per cell where F may be non-zero; F of every
 // other cell is zero in the reference even if the array still holds an old
 // value (forcing's "F = 0 where O == 0", Forcing.hpp:141-151).
+//
+// Reduced-precision storage (TDLB_PREC_F16 / TDLB_PREC_BF16): the lazy phase
+// keeps the populations as 16-bit scaled deviations (f_d - w_d rho0) / w_d and
+// computes in float; the explicit Q is a float array M (the FP32 reference
+// layout), written by materialise and read by the first collision of a run.
+// The per-call operations (collision, streaming, ...) need fp32 or fp64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -84,6 +90,24 @@ struct DevPoints {
     }
 };
 
+// Stencil cells of a point list with a moments cache: the sorted unique cells,
+// and for every point the 9 positions of its stencil cells in that list.
+struct IndexedSet {
+    uint32_t* cells = nullptr;
+    int32_t* sidx = nullptr;
+    void* cache = nullptr;
+    int64_t n_cells = 0, n_points = 0;
+    void release() {
+        if (cells) (void)hipFree(cells);
+        if (sidx) (void)hipFree(sidx);
+        if (cache) (void)hipFree(cache);
+        cells = nullptr;
+        sidx = nullptr;
+        cache = nullptr;
+        n_cells = n_points = 0;
+    }
+};
+
 // Host-side copy of points in the CU's type, structure of arrays.
 struct HostPoints {
     std::vector<int32_t> i, j, k;
@@ -107,12 +131,15 @@ struct tdlb_cu {
     tdlb_flow flow{};
     Grid g{};
     int64_t xg = 0, yg = 0, zg = 0;
-    size_t tsize = 4;
+    size_t tsize = 4;  // arithmetic type: F, Nu, points, explicit Q
+    size_t ssize = 4;  // stored population type
     hipStream_t stream = nullptr;
     int device = 0;
 
     void* Q[2] = {nullptr, nullptr};
     int cur = 0;
+    void* M = nullptr;   // explicit Q of the reduced-precision storage (float)
+    float rho0 = 1.0f;   // rest density of the deviation storage
     void* F = nullptr;
     void* Nu = nullptr;
     uint8_t* O = nullptr;
@@ -132,6 +159,8 @@ struct tdlb_cu {
     bool bound = false;
     tdlb_rushton_params rp{};
     DevPoints pts;                       // static points [0, n_static), blades after
+    IndexedSet static_set;               // stencil cells of the static points
+    void* rot_cache = nullptr;           // 9 moment entries per blade point
     int64_t n_static = 0, n_rot = 0;
     std::vector<int64_t> static_levels;  // CSR offsets into [0, n_static)
     std::vector<int64_t> rot_levels;     // CSR offsets into [n_static, n_static + n_rot)
@@ -139,6 +168,7 @@ struct tdlb_cu {
     double alfa = 0.97, beta = 1.9;
 
     DevPoints api_pts;  // points of the generic forcing call
+    IndexedSet api_set;
 
     // event timing of tdlb_cu_run (tdlb_cu_profile)
     bool prof = false;
@@ -146,12 +176,16 @@ struct tdlb_cu {
     std::vector<std::pair<int, int>> coll, force;
     size_t ev_used = 0;
 
-    size_t q_bytes() const { return (size_t)g.size * kQ * tsize; }
+    size_t q_bytes() const { return (size_t)g.size * kQ * tsize; }   // Q as the reference holds it
+    size_t qs_bytes() const { return (size_t)g.size * kQ * ssize; }  // one population lattice
     size_t f_bytes() const { return (size_t)g.size * 3 * tsize; }
     size_t nu_bytes() const { return (size_t)g.size * tsize; }
     size_t mask_bytes() const { return mask_words * sizeof(uint64_t); }
     bool simple() const { return prm.streaming == TDLB_STREAM_SIMPLE; }
-    int prec() const { return prm.precision; }
+    bool shifted() const { return prm.precision == TDLB_PREC_F16 || prm.precision == TDLB_PREC_BF16; }
+    int prec() const { return prm.precision; }            // kernels on the stored populations
+    int eprec() const { return prm.precision == TDLB_PREC_F64 ? 1 : 0; }  // kernels on the explicit Q
+    void* eq() const { return shifted() ? M : Q[cur]; }   // the explicit Q array
 
     CollideArgs cargs() const {
         CollideArgs c;
@@ -171,12 +205,15 @@ int set_device(tdlb_cu* cu) {
 }
 
 void release(tdlb_cu* cu) {
-    void* ptrs[] = {cu->Q[0], cu->Q[1], cu->F, cu->Nu, cu->O, cu->fmask[0], cu->fmask[1],
+    void* ptrs[] = {cu->Q[0], cu->Q[1], cu->M, cu->F, cu->Nu, cu->O, cu->fmask[0], cu->fmask[1],
                     cu->gmask, cu->static_bits, cu->static_base};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     cu->pts.release();
     cu->api_pts.release();
+    cu->static_set.release();
+    cu->api_set.release();
+    if (cu->rot_cache) (void)hipFree(cu->rot_cache);
     for (hipEvent_t e : cu->events) (void)hipEventDestroy(e);
     cu->events.clear();
     if (cu->stream) (void)hipStreamDestroy(cu->stream);
@@ -203,16 +240,28 @@ int materialise(tdlb_cu* cu) {
     const auto c = cu->cargs();
     if (cu->simple()) {
         void* src = cu->Q[cu->cur];
-        void* dst = cu->Q[1 - cu->cur];
-        HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_MOM_PULL, src, dst, cu->F, cu->Nu,
-                             cu->fmask[cu->fm], cu->g, c, cu->stream));
-        HIPCHK(launch_ghost_fill((int)cu->tsize, cu->prm.layout, src, dst, cu->g, cu->stream));
-        cu->cur = 1 - cu->cur;
+        void* dst = cu->shifted() ? cu->M : cu->Q[1 - cu->cur];
+        HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_MOM_PULL, src, dst, dst, cu->F, cu->Nu,
+                             cu->fmask[cu->fm], cu->g, c, cu->rho0, cu->stream));
+        if (cu->shifted()) {
+            HIPCHK(launch_ghost_fill_decode(cu->prec(), cu->prm.layout, src, dst, cu->g, cu->rho0, cu->stream));
+        } else {
+            HIPCHK(launch_ghost_fill((int)cu->ssize, cu->prm.layout, src, dst, cu->g, cu->stream));
+            cu->cur = 1 - cu->cur;
+        }
     } else {
-        HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_MOM, cu->even ? 1 : 0, cu->Q[0], cu->F, cu->Nu,
-                               cu->fmask[cu->fm], cu->g, c, cu->stream));
+        HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_MOM, cu->even ? 1 : 0, cu->Q[0], cu->eq(), cu->F,
+                               cu->Nu, cu->fmask[cu->fm], cu->g, c, cu->rho0, cu->stream));
     }
     cu->phase = PH_EXPLICIT;
+    return TDLB_OK;
+}
+
+// the per-call operations work on the reference's Q directly
+int need_full_precision(const tdlb_cu* cu, const char* what) {
+    if (cu->shifted())
+        return fail(TDLB_ERR_UNSUPPORTED, std::string(what) + " needs precision fp32 or fp64 (the 16-bit "
+                                          "population storage runs through tdlb_cu_run only)");
     return TDLB_OK;
 }
 
@@ -310,22 +359,87 @@ int upload_points(tdlb_cu* cu, DevPoints& dp, int64_t offset, const HostPoints& 
     return TDLB_OK;
 }
 
-// One forcing pass.  Levels: list of (begin, end) point ranges run in order.
-// prepare: point ranges whose stencil cells may be missing from the previous
-// occupancy mask.  base: bits the new mask starts from.
-int forcing_pass(tdlb_cu* cu, const DevPoints& dp, const std::vector<std::pair<int64_t, int64_t>>& levels,
+// Build the unique-cell moment cache description of points (in device order)
+// and upload it; sidx entries are indexed by the point's position + offset.
+int build_indexed_set(tdlb_cu* cu, const tdlb_ib_point* p, const std::vector<int64_t>& order, IndexedSet& set) {
+    set.release();
+    const int64_t n = (int64_t)order.size();
+    if (n == 0) return TDLB_OK;
+    std::vector<uint32_t> all(9 * n);
+    uint64_t cells[9];
+    const int64_t gh = cu->prm.ghost;
+    for (int64_t a = 0; a < n; a++) {
+        const tdlb_ib_point& q = p[order[a]];
+        stencil_cells(cu, q.i + gh, q.j + gh, q.k + gh, cells);
+        for (int s = 0; s < 9; s++) all[9 * a + s] = (uint32_t)cells[s];
+    }
+    std::vector<uint32_t> uniq(all);
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    std::vector<int32_t> sidx(9 * n);
+    for (int64_t e = 0; e < 9 * n; e++)
+        sidx[e] = (int32_t)(std::lower_bound(uniq.begin(), uniq.end(), all[e]) - uniq.begin());
+    set.n_cells = (int64_t)uniq.size();
+    set.n_points = n;
+    if (hipMalloc((void**)&set.cells, uniq.size() * 4) != hipSuccess ||
+        hipMalloc((void**)&set.sidx, sidx.size() * 4) != hipSuccess ||
+        hipMalloc(&set.cache, uniq.size() * 4 * cu->tsize) != hipSuccess)
+        return fail(TDLB_ERR_NOMEM, "IB moment cache");
+    HIPCHK(hipMemcpyAsync(set.cells, uniq.data(), uniq.size() * 4, hipMemcpyHostToDevice, cu->stream));
+    HIPCHK(hipMemcpyAsync(set.sidx, sidx.data(), sidx.size() * 4, hipMemcpyHostToDevice, cu->stream));
+    HIPCHK(hipStreamSynchronize(cu->stream));
+    return TDLB_OK;
+}
+
+// Point ranges of one forcing pass sharing a source of stencil moments.
+struct ForceSegment {
+    const IndexedSet* set = nullptr;  // unique-cell cache (sidx by point index)
+    void* direct_cache = nullptr;     // or 9 entries per point from cache_base
+    int64_t cache_base = 0, n_points = 0;
+    std::vector<std::pair<int64_t, int64_t>> levels;
+};
+
+// One forcing pass: levels of every segment in order.  prepare: point ranges
+// whose stencil cells may be missing from the previous occupancy mask.  base:
+// bits the new mask starts from.
+int forcing_pass(tdlb_cu* cu, const DevPoints& dp, const std::vector<ForceSegment>& segs,
                  const std::vector<std::pair<int64_t, int64_t>>& prepare, const uint64_t* base, int momsrc) {
     uint64_t* prev = cu->fmask[cu->fm];
     uint64_t* next = cu->fmask[1 - cu->fm];
+    const int even = cu->even ? 1 : 0;
+    // stored moments live in the explicit Q, populations in the lattice
+    const bool stored = momsrc == MOMSRC_STORED;
+    const void* Qsrc = stored ? cu->eq() : cu->Q[cu->cur];
+    const int qp = stored ? cu->eprec() : cu->prec();
     HIPCHK(launch_copy(next, base, cu->mask_bytes(), cu->stream));
     for (auto& r : prepare)
-        HIPCHK(launch_force_prepare(cu->prec(), dp.d, r.first, r.second - r.first, cu->F, prev, next, cu->g,
+        HIPCHK(launch_force_prepare(cu->eprec(), dp.d, r.first, r.second - r.first, cu->F, prev, next, cu->g,
                                     cu->stream));
-    const void* Qsrc = cu->Q[cu->cur];
-    for (auto& r : levels)
-        HIPCHK(launch_force_level(cu->prec(), cu->prm.layout, cu->prm.streaming, momsrc, cu->even ? 1 : 0, dp.d,
-                                  r.first, r.second - r.first, cu->F, Qsrc, cu->g, cu->alfa, cu->beta,
-                                  cu->stream));
+    // stencils wrap onto themselves below 3 interior cells: keep the plain kernel there
+    const bool tiny = cu->g.xg1 < 3 || cu->g.zg1 < 3;
+    if (!tiny) {
+        for (auto& sg : segs) {
+            if (sg.set)
+                HIPCHK(launch_ib_moments_cells(qp, cu->prm.layout, cu->prm.streaming, momsrc, even, sg.set->cells,
+                                               sg.set->n_cells, sg.set->cache, Qsrc, cu->g, cu->rho0, cu->stream));
+            else
+                HIPCHK(launch_ib_moments_points(qp, cu->prm.layout, cu->prm.streaming, momsrc, even, dp.d,
+                                                sg.cache_base, sg.n_points, sg.direct_cache, Qsrc, cu->g, cu->rho0,
+                                                cu->stream));
+        }
+    }
+    for (auto& sg : segs)
+        for (auto& r : sg.levels) {
+            if (tiny)
+                HIPCHK(launch_force_level(qp, cu->prm.layout, cu->prm.streaming, momsrc, even, dp.d, r.first,
+                                          r.second - r.first, cu->F, Qsrc, cu->g, cu->alfa, cu->beta, cu->rho0,
+                                          cu->stream));
+            else
+                HIPCHK(launch_force_level_cached(cu->eprec(), dp.d, r.first, r.second - r.first,
+                                                 sg.set ? sg.set->sidx : nullptr, sg.cache_base,
+                                                 sg.set ? sg.set->cache : sg.direct_cache, cu->F, cu->g, cu->alfa,
+                                                 cu->beta, cu->stream));
+        }
     if (cu->o_dirty) {
         HIPCHK(launch_zero_interior_o(cu->O, cu->g, cu->stream));
         cu->o_dirty = false;
@@ -335,14 +449,20 @@ int forcing_pass(tdlb_cu* cu, const DevPoints& dp, const std::vector<std::pair<i
 }
 
 int bound_forcing(tdlb_cu* cu, int momsrc, bool prepare_static) {
-    std::vector<std::pair<int64_t, int64_t>> levels, prep;
+    std::vector<ForceSegment> segs(2);
+    segs[0].set = &cu->static_set;
+    segs[0].n_points = cu->n_static;
     for (size_t l = 0; l + 1 < cu->static_levels.size(); l++)
-        levels.push_back({cu->static_levels[l], cu->static_levels[l + 1]});
+        segs[0].levels.push_back({cu->static_levels[l], cu->static_levels[l + 1]});
+    segs[1].direct_cache = cu->rot_cache;
+    segs[1].cache_base = cu->n_static;
+    segs[1].n_points = cu->n_rot;
     for (size_t l = 0; l + 1 < cu->rot_levels.size(); l++)
-        levels.push_back({cu->rot_levels[l], cu->rot_levels[l + 1]});
+        segs[1].levels.push_back({cu->rot_levels[l], cu->rot_levels[l + 1]});
+    std::vector<std::pair<int64_t, int64_t>> prep;
     if (prepare_static) prep.push_back({0, cu->n_static});
     prep.push_back({cu->n_static, cu->n_static + cu->n_rot});
-    CHK(forcing_pass(cu, cu->pts, levels, prep, cu->static_base, momsrc));
+    CHK(forcing_pass(cu, cu->pts, segs, prep, cu->static_base, momsrc));
     cu->static_in_prev = true;
     return TDLB_OK;
 }
@@ -375,7 +495,7 @@ int tdlb_device_count(int* count) {
 
 int tdlb_flow_calc_nu_re_m(tdlb_flow* f, int32_t outer_radius, int32_t precision) {
     if (!f) return fail(TDLB_ERR_INVALID, "null flow");
-    if (precision == TDLB_PREC_F32) {
+    if (precision == TDLB_PREC_F32 || precision == TDLB_PREC_F16 || precision == TDLB_PREC_BF16) {
         const float reM = (float)f->re_m_nondimensional, uav = (float)f->uav;
         const float Re_m = reM * M_PI / 2.0;
         const float nu = uav * static_cast<float>(outer_radius) / Re_m;
@@ -398,7 +518,7 @@ int tdlb_cu_create(const tdlb_cu_params* p, const tdlb_flow* f, tdlb_cu_t* out) 
     if (p->layout != TDLB_LAYOUT_IJKL && p->layout != TDLB_LAYOUT_LIJK) return fail(TDLB_ERR_INVALID, "layout");
     if (p->streaming != TDLB_STREAM_SIMPLE && p->streaming != TDLB_STREAM_ESOTWIST)
         return fail(TDLB_ERR_INVALID, "streaming");
-    if (p->precision != TDLB_PREC_F32 && p->precision != TDLB_PREC_F64)
+    if (p->precision < TDLB_PREC_F32 || p->precision > TDLB_PREC_BF16)
         return fail(TDLB_ERR_UNSUPPORTED, "precision");
     if (f->collision != 0) return fail(TDLB_ERR_UNSUPPORTED, "only the EgglesSomers collision is implemented");
     const int64_t xg = p->x + 2, yg = p->y + 2, zg = p->z + 2;
@@ -414,7 +534,9 @@ int tdlb_cu_create(const tdlb_cu_params* p, const tdlb_flow* f, tdlb_cu_t* out) 
     cu->g.yz = (uint32_t)(yg * zg);
     cu->g.mask_w = (uint32_t)((zg + 63) / 64);
     cu->g.size = size;
-    cu->tsize = p->precision == TDLB_PREC_F32 ? 4 : 8;
+    cu->tsize = p->precision == TDLB_PREC_F64 ? 8 : 4;
+    cu->ssize = p->precision == TDLB_PREC_F64 ? 8 : p->precision == TDLB_PREC_F32 ? 4 : 2;
+    cu->rho0 = (float)f->initial_rho;
     cu->mask_words = (size_t)xg * yg * cu->g.mask_w;
     cu->device = p->device_id < 0 ? 0 : p->device_id;
     cu->alfa = f->alpha;
@@ -430,7 +552,8 @@ int tdlb_cu_create(const tdlb_cu_params* p, const tdlb_flow* f, tdlb_cu_t* out) 
         return bad(TDLB_ERR_HIP, "hipStreamCreate failed");
     const int nq = cu->simple() ? 2 : 1;
     for (int b = 0; b < nq; b++)
-        if (hipMalloc(&cu->Q[b], cu->q_bytes()) != hipSuccess) return bad(TDLB_ERR_NOMEM, "hipMalloc Q");
+        if (hipMalloc(&cu->Q[b], cu->qs_bytes()) != hipSuccess) return bad(TDLB_ERR_NOMEM, "hipMalloc Q");
+    if (cu->shifted() && hipMalloc(&cu->M, cu->q_bytes()) != hipSuccess) return bad(TDLB_ERR_NOMEM, "hipMalloc M");
     if (hipMalloc(&cu->F, cu->f_bytes()) != hipSuccess) return bad(TDLB_ERR_NOMEM, "hipMalloc F");
     if (hipMalloc(&cu->Nu, cu->nu_bytes()) != hipSuccess) return bad(TDLB_ERR_NOMEM, "hipMalloc Nu");
     if (hipMalloc((void**)&cu->O, size) != hipSuccess) return bad(TDLB_ERR_NOMEM, "hipMalloc O");
@@ -446,7 +569,8 @@ int tdlb_cu_create(const tdlb_cu_params* p, const tdlb_flow* f, tdlb_cu_t* out) 
         launch_zero(cu->gmask, cu->mask_bytes(), cu->stream) != hipSuccess)
         return bad(TDLB_ERR_HIP, "initial fill failed");
     for (int b = 0; b < nq; b++)
-        if (launch_zero(cu->Q[b], cu->q_bytes(), cu->stream) != hipSuccess) return bad(TDLB_ERR_HIP, "Q fill failed");
+        if (launch_zero(cu->Q[b], cu->qs_bytes(), cu->stream) != hipSuccess) return bad(TDLB_ERR_HIP, "Q fill failed");
+    if (cu->M && launch_zero(cu->M, cu->q_bytes(), cu->stream) != hipSuccess) return bad(TDLB_ERR_HIP, "M fill failed");
     if (hipStreamSynchronize(cu->stream) != hipSuccess) return bad(TDLB_ERR_HIP, "synchronise failed");
     *out = cu;
     return TDLB_OK;
@@ -474,8 +598,14 @@ int tdlb_cu_initialise(tdlb_cu_t cu, double rho) {
     if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
     CHK(set_device(cu));
     const double r = to_prec(cu, rho);
-    const int nq = cu->simple() ? 2 : 1;
-    for (int b = 0; b < nq; b++) HIPCHK(launch_fill_q(cu->prec(), cu->prm.layout, cu->Q[b], cu->g, r, cu->stream));
+    if (cu->shifted()) {
+        cu->rho0 = (float)r;
+        HIPCHK(launch_fill_q(cu->eprec(), cu->prm.layout, cu->M, cu->g, r, cu->stream));
+    } else {
+        const int nq = cu->simple() ? 2 : 1;
+        for (int b = 0; b < nq; b++)
+            HIPCHK(launch_fill_q(cu->prec(), cu->prm.layout, cu->Q[b], cu->g, r, cu->stream));
+    }
     HIPCHK(launch_zero(cu->F, cu->f_bytes(), cu->stream));
     HIPCHK(launch_zero(cu->Nu, cu->nu_bytes(), cu->stream));
     HIPCHK(launch_zero(cu->O, cu->g.size, cu->stream));
@@ -491,34 +621,39 @@ int tdlb_cu_initialise(tdlb_cu_t cu, double rho) {
 
 int tdlb_cu_collision(tdlb_cu_t cu) {
     if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    CHK(need_full_precision(cu, "collision"));
     CHK(set_device(cu));
     CHK(materialise(cu));
     const auto c = cu->cargs();
+    void* q = cu->Q[cu->cur];
     if (cu->simple())
-        HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_COLL, cu->Q[cu->cur], cu->Q[cu->cur], cu->F, cu->Nu,
-                             cu->fmask[cu->fm], cu->g, c, cu->stream));
+        HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_COLL, q, q, q, cu->F, cu->Nu, cu->fmask[cu->fm],
+                             cu->g, c, cu->rho0, cu->stream));
     else
-        HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_COLL, cu->even ? 1 : 0, cu->Q[0], cu->F, cu->Nu,
-                               cu->fmask[cu->fm], cu->g, c, cu->stream));
+        HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_COLL, cu->even ? 1 : 0, q, q, cu->F, cu->Nu,
+                               cu->fmask[cu->fm], cu->g, c, cu->rho0, cu->stream));
     return TDLB_OK;
 }
 
 int tdlb_cu_moments(tdlb_cu_t cu) {
     if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    CHK(need_full_precision(cu, "moments"));
     CHK(set_device(cu));
     CHK(materialise(cu));
     const auto c = cu->cargs();
+    void* q = cu->Q[cu->cur];
     if (cu->simple())
-        HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_MOM, cu->Q[cu->cur], cu->Q[cu->cur], cu->F, cu->Nu,
-                             cu->fmask[cu->fm], cu->g, c, cu->stream));
+        HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_MOM, q, q, q, cu->F, cu->Nu, cu->fmask[cu->fm], cu->g,
+                             c, cu->rho0, cu->stream));
     else
-        HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_MOM, cu->even ? 1 : 0, cu->Q[0], cu->F, cu->Nu,
-                               cu->fmask[cu->fm], cu->g, c, cu->stream));
+        HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_MOM, cu->even ? 1 : 0, q, q, cu->F, cu->Nu,
+                               cu->fmask[cu->fm], cu->g, c, cu->rho0, cu->stream));
     return TDLB_OK;
 }
 
 int tdlb_cu_stream_push(tdlb_cu_t cu) {
     if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    CHK(need_full_precision(cu, "streaming"));
     CHK(set_device(cu));
     CHK(materialise(cu));
     if (!cu->simple()) {
@@ -532,6 +667,7 @@ int tdlb_cu_stream_push(tdlb_cu_t cu) {
 
 int tdlb_cu_stream_pull(tdlb_cu_t cu) {
     if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    CHK(need_full_precision(cu, "streaming"));
     CHK(set_device(cu));
     CHK(materialise(cu));
     if (!cu->simple()) {
@@ -545,6 +681,7 @@ int tdlb_cu_stream_pull(tdlb_cu_t cu) {
 
 int tdlb_cu_bounce_back(tdlb_cu_t cu) {
     if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    CHK(need_full_precision(cu, "bounce-back"));
     CHK(set_device(cu));
     CHK(materialise(cu));
     if (!cu->simple()) return TDLB_OK;  // Boundary.hpp:13-14
@@ -554,6 +691,7 @@ int tdlb_cu_bounce_back(tdlb_cu_t cu) {
 
 int tdlb_cu_forcing(tdlb_cu_t cu, const tdlb_ib_point* points, int64_t n, double alfa, double beta) {
     if (!cu || (n > 0 && !points) || n < 0) return fail(TDLB_ERR_INVALID, "bad arguments");
+    CHK(need_full_precision(cu, "forcing"));
     CHK(set_device(cu));
     CHK(check_points(cu, points, n));
     CHK(materialise(cu));
@@ -563,13 +701,17 @@ int tdlb_cu_forcing(tdlb_cu_t cu, const tdlb_ib_point* points, int64_t n, double
     to_host_points(cu, points, order, h);
     CHK(cu->api_pts.reserve(n, cu->tsize));
     CHK(upload_points(cu, cu->api_pts, 0, h));
-    std::vector<std::pair<int64_t, int64_t>> levels, prep;
-    for (size_t l = 0; l + 1 < offsets.size(); l++) levels.push_back({offsets[l], offsets[l + 1]});
+    CHK(build_indexed_set(cu, points, order, cu->api_set));
+    std::vector<ForceSegment> segs(1);
+    segs[0].set = &cu->api_set;
+    segs[0].n_points = n;
+    for (size_t l = 0; l + 1 < offsets.size(); l++) segs[0].levels.push_back({offsets[l], offsets[l + 1]});
+    std::vector<std::pair<int64_t, int64_t>> prep;
     if (n > 0) prep.push_back({0, n});
     const double a0 = cu->alfa, b0 = cu->beta;
     cu->alfa = alfa;
     cu->beta = beta;
-    int r = forcing_pass(cu, cu->api_pts, levels, prep, cu->gmask, MOMSRC_STORED);
+    int r = forcing_pass(cu, cu->api_pts, segs, prep, cu->gmask, MOMSRC_STORED);
     cu->alfa = a0;
     cu->beta = b0;
     cu->static_in_prev = false;
@@ -604,7 +746,7 @@ int tdlb_cu_download(tdlb_cu_t cu, int field, void* host, size_t bytes) {
         if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("download F: ") + hipGetErrorString(e));
         return TDLB_OK;
     }
-    const void* src = field == TDLB_FIELD_Q ? cu->Q[cu->cur] : field == TDLB_FIELD_NU ? cu->Nu : (const void*)cu->O;
+    const void* src = field == TDLB_FIELD_Q ? cu->eq() : field == TDLB_FIELD_NU ? cu->Nu : (const void*)cu->O;
     HIPCHK(hipMemcpyAsync(host, src, need, hipMemcpyDeviceToHost, cu->stream));
     HIPCHK(hipStreamSynchronize(cu->stream));
     return TDLB_OK;
@@ -616,7 +758,7 @@ int tdlb_cu_upload(tdlb_cu_t cu, int field, const void* host, size_t bytes) {
     if (!host || bytes != need) return fail(TDLB_ERR_INVALID, "upload: buffer size must be " + std::to_string(need));
     CHK(set_device(cu));
     CHK(materialise(cu));
-    void* dst = field == TDLB_FIELD_Q ? cu->Q[cu->cur] : field == TDLB_FIELD_F ? cu->F
+    void* dst = field == TDLB_FIELD_Q ? cu->eq() : field == TDLB_FIELD_F ? cu->F
               : field == TDLB_FIELD_NU ? cu->Nu : (void*)cu->O;
     HIPCHK(hipMemcpyAsync(dst, host, need, hipMemcpyHostToDevice, cu->stream));
     if (field == TDLB_FIELD_F) {
@@ -824,6 +966,11 @@ int tdlb_cu_bind_rushton(tdlb_cu_t cu, const tdlb_rushton_params* p, uint64_t st
     }
     CHK(cu->pts.reserve(cu->n_static + cu->n_rot, cu->tsize));
     CHK(upload_points(cu, cu->pts, 0, hs));
+    CHK(build_indexed_set(cu, stat.data(), order, cu->static_set));
+    if (cu->rot_cache) (void)hipFree(cu->rot_cache);
+    cu->rot_cache = nullptr;
+    if (cu->n_rot > 0 && hipMalloc(&cu->rot_cache, (size_t)cu->n_rot * 9 * 4 * cu->tsize) != hipSuccess)
+        return fail(TDLB_ERR_NOMEM, "blade moment cache");
 
     cu->blade.n_blades = p->n_blades;
     cu->blade.j_lo = (int32_t)p->blade_plane_lo;
@@ -884,27 +1031,30 @@ int tdlb_cu_run(tdlb_cu_t cu, int64_t nsteps, uint64_t* step, double* angle) {
         CHK(gen_blades(cu, *angle, delta));                     // updateRotatingGeometry
         const int c0 = prof_mark(cu);
         if (cu->phase == PH_EXPLICIT) {
+            void* q = cu->Q[cu->cur];
+            void* e = cu->eq();
             if (cu->simple()) {
-                // settle the ghost layer once (bounce-back rims), share it with the other lattice
-                HIPCHK(launch_bounce_back((int)cu->tsize, cu->prm.layout, cu->Q[cu->cur], cu->g, cu->stream));
-                HIPCHK(launch_ghost_copy((int)cu->tsize, cu->prm.layout, cu->Q[cu->cur], cu->Q[1 - cu->cur],
-                                         cu->g, cu->stream));
-                HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_COLL, cu->Q[cu->cur], cu->Q[cu->cur], cu->F,
-                                     cu->Nu, cu->fmask[cu->fm], cu->g, c, cu->stream));
+                // settle the ghost layer once (bounce-back rims); the 16-bit lattices never read theirs
+                HIPCHK(launch_bounce_back((int)cu->tsize, cu->prm.layout, e, cu->g, cu->stream));
+                if (!cu->shifted())
+                    HIPCHK(launch_ghost_copy((int)cu->ssize, cu->prm.layout, q, cu->Q[1 - cu->cur], cu->g,
+                                             cu->stream));
+                HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_COLL, q, q, e, cu->F, cu->Nu,
+                                     cu->fmask[cu->fm], cu->g, c, cu->rho0, cu->stream));
             } else {
-                HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_COLL, cu->even ? 1 : 0, cu->Q[0], cu->F,
-                                       cu->Nu, cu->fmask[cu->fm], cu->g, c, cu->stream));
+                HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_COLL, cu->even ? 1 : 0, q, e, cu->F, cu->Nu,
+                                       cu->fmask[cu->fm], cu->g, c, cu->rho0, cu->stream));
                 cu->even = !cu->even;
             }
             cu->phase = PH_LAZY;
         } else {
             if (cu->simple()) {
                 HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_FUSED, cu->Q[cu->cur], cu->Q[1 - cu->cur],
-                                     cu->F, cu->Nu, cu->fmask[cu->fm], cu->g, c, cu->stream));
+                                     nullptr, cu->F, cu->Nu, cu->fmask[cu->fm], cu->g, c, cu->rho0, cu->stream));
                 cu->cur = 1 - cu->cur;
             } else {
-                HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_FUSED, cu->even ? 1 : 0, cu->Q[0], cu->F,
-                                       cu->Nu, cu->fmask[cu->fm], cu->g, c, cu->stream));
+                HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_FUSED, cu->even ? 1 : 0, cu->Q[0], nullptr,
+                                       cu->F, cu->Nu, cu->fmask[cu->fm], cu->g, c, cu->rho0, cu->stream));
                 cu->even = !cu->even;
             }
         }

@@ -3,12 +3,14 @@
 // moments:   tdlbcpp/src/CollisionEgglesSomers.hpp:185-256
 // collision: tdlbcpp/src/CollisionEgglesSomers.hpp:26-181 (+ velocity QVec.hpp:125-134)
 //
-// The reference is compiled with T = float and double literals, so several
-// sub-expressions are evaluated in double and rounded back to T.  To give
-// results identical to the reference, every expression below spells those
-// promotions out as (D) casts and keeps the reference's left-to-right order of
-// terms.  All code that includes this header is compiled with
-// -ffp-contract=off (no fused multiply-add).
+// T is the type of the stored values, P the type the reference's double
+// literals promote to.  With P = double (the reference arithmetic) the reference
+// is compiled with T = float and double literals, so several sub-expressions
+// are evaluated in double and rounded back to T; every expression below spells
+// those promotions out as (D) casts and keeps the reference's left-to-right order
+// of terms, so that results are identical to the reference.  With P = float the
+// same expressions run entirely in FP32 (the reduced-precision storage modes).
+// All code that includes this header is compiled with -ffp-contract=off.
 //
 // One deliberate rewrite: the reference divides each of the 18 relaxed moments
 // by 24.0 in double and rounds to float.  For T = float we multiply by the
@@ -31,16 +33,15 @@ struct CollideParams {
     int use_les;
 };
 
-template <typename T>
+template <typename T, typename D = double>
 TDLB_HD inline CollideParams<T> make_collide_params(T nu, T cs0, T g3, int use_les) {
-    using D = double;
     CollideParams<T> p;
     p.nu = nu;
     p.cs0 = cs0;
     p.g3 = g3;
     p.use_les = use_les;
-    p.b0 = (T)(1.0 / (1.0 + (D)(T(6) * nu)));
-    p.c0 = (T)(1.0 - (D)(T(6) * nu));
+    p.b0 = (T)(D(1.0) / (D(1.0) + (D)(T(6) * nu)));
+    p.c0 = (T)(D(1.0) - (D)(T(6) * nu));
     return p;
 }
 
@@ -49,19 +50,22 @@ TDLB_HD inline CollideParams<T> make_collide_params(T nu, T cs0, T g3, int use_l
 TDLB_HD inline float sqrt_abs(float x) { return sqrtf(fabsf(x)); }
 TDLB_HD inline double sqrt_abs(double x) { return sqrt(fabs(x)); }
 
-template <typename T>
-TDLB_HD inline T div24(T x) {
-    return (T)((double)x / 24.0);
-}
+template <typename T, typename D>
+struct Div24 {
+    TDLB_HD static T apply(T x) { return (T)((D)x / D(24.0)); }
+};
 template <>
-TDLB_HD inline float div24<float>(float x) {
-    return (float)((double)x * (1.0 / 24.0));
-}
+struct Div24<float, double> {
+    TDLB_HD static float apply(float x) { return (float)((double)x * (1.0 / 24.0)); }
+};
+template <>
+struct Div24<float, float> {
+    TDLB_HD static float apply(float x) { return x * (1.0f / 24.0f); }
+};
 
 // Eighteen moments of eighteen populations; same term order as the reference.
-template <typename T>
+template <typename T, typename D = double>
 TDLB_HD inline void es_moments(const T* f, T* m) {
-    using D = double;
     const T two = T(2), three = T(3);
     m[M01] = f[Q01] + f[Q03] + f[Q02] + f[Q04] + f[Q05] + f[Q06] + f[Q07] + f[Q14] + f[Q08] + f[Q13] + f[Q09]
            + f[Q16] + f[Q10] + f[Q15] + f[Q11] + f[Q18] + f[Q12] + f[Q17];
@@ -90,7 +94,7 @@ TDLB_HD inline void es_moments(const T* f, T* m) {
     m[M16] = -three * f[Q09] - three * f[Q16] + three * f[Q10] + three * f[Q15] + three * f[Q11]
            + three * f[Q18] - three * f[Q12] - three * f[Q17];
     // fourth order: double literals 0.5 / 1.5 make these sums double
-    const D h = 0.5, o = 1.5;
+    const D h = D(0.5), o = D(1.5);
     m[M17] = (T)(h * (D)f[Q01] + h * (D)f[Q03] + h * (D)f[Q02] + h * (D)f[Q04] - (D)f[Q05] - (D)f[Q06]
                  - (D)f[Q07] - (D)f[Q14] - (D)f[Q08] - (D)f[Q13] + h * (D)f[Q09] + h * (D)f[Q16]
                  + h * (D)f[Q10] + h * (D)f[Q15] + h * (D)f[Q11] + h * (D)f[Q18] + h * (D)f[Q12]
@@ -112,18 +116,18 @@ TDLB_HD inline void es_rho_j(const T* f, T& rho, T& jx, T& jy, T& jz) {
 
 // Collision of one cell: m (moments) in, p (post-collision populations) out.
 // nuc is the cell's eddy viscosity: read (previous step) and written with LES.
-template <typename T>
-TDLB_HD inline void es_collide(const T* m, T fx, T fy, T fz, T& nuc, const CollideParams<T>& P, T* p) {
-    using D = double;
-    const D inv_rho = 1.0 / (D)m[M01];
-    const T ux = (T)(inv_rho * ((D)m[M02] + 0.5 * (D)fx));
-    const T uy = (T)(inv_rho * ((D)m[M03] + 0.5 * (D)fy));
-    const T uz = (T)(inv_rho * ((D)m[M04] + 0.5 * (D)fz));
+template <typename T, typename D = double>
+TDLB_HD inline void es_collide(const T* m, T fx, T fy, T fz, T& nuc, const CollideParams<T>& cp, T* p) {
+    const D one = D(1.0), half = D(0.5), two = D(2.0);
+    const D inv_rho = one / (D)m[M01];
+    const T ux = (T)(inv_rho * ((D)m[M02] + half * (D)fx));
+    const T uy = (T)(inv_rho * ((D)m[M03] + half * (D)fy));
+    const T uz = (T)(inv_rho * ((D)m[M04] + half * (D)fz));
 
-    T b = P.b0, c = P.c0;
-    if (P.use_les) {
-        const T fct = (T)(3.0 / ((D)m[M01] * (1.0 + 6.0 * (D)(nuc + P.nu))));
-        const D hfx = 0.5 * (D)fx, hfy = 0.5 * (D)fy, hfz = 0.5 * (D)fz;
+    T b = cp.b0, c = cp.c0;
+    if (cp.use_les) {
+        const T fct = (T)(D(3.0) / ((D)m[M01] * (one + D(6.0) * (D)(nuc + cp.nu))));
+        const D hfx = half * (D)fx, hfy = half * (D)fy, hfz = half * (D)fz;
         const T dudx = (T)((D)fct * (((D)m[M02] + hfx * (D)ux) - (D)m[M05]));
         const T dvdy = (T)((D)fct * (((D)m[M03] + hfy * (D)uy) - (D)m[M07]));
         const T dwdz = (T)((D)fct * (((D)m[M04] + hfz * (D)uz) - (D)m[M10]));
@@ -134,16 +138,16 @@ TDLB_HD inline void es_collide(const T* m, T fx, T fy, T fz, T& nuc, const Colli
         const T syz = (T)((D)fct2 * (((D)m[M04] + hfz * (D)uy) - (D)m[M09]));
         const T sh_t = T(2) * dudx * dudx + T(2) * dvdy * dvdy + T(2) * dwdz * dwdz + sxy * sxy + sxz * sxz
                      + syz * syz;
-        const T sh = (T)((D)sh_t - (2.0 / 3.0) * (D)divv * (D)divv);
-        nuc = P.cs0 * P.cs0 * sqrt_abs(sh);
-        const T nut = nuc + P.nu;
-        b = (T)(1.0 / (1.0 + (D)(T(6) * nut)));
-        c = (T)(1.0 - (D)(T(6) * nut));
+        const T sh = (T)((D)sh_t - (D(2.0) / D(3.0)) * (D)divv * (D)divv);
+        nuc = cp.cs0 * cp.cs0 * sqrt_abs(sh);
+        const T nut = nuc + cp.nu;
+        b = (T)(one / (one + (D)(T(6) * nut)));
+        c = (T)(one - (D)(T(6) * nut));
     }
 
-    const D jx2 = 2.0 * ((D)m[M02] + 0.5 * (D)fx);
-    const D jy2 = 2.0 * ((D)m[M03] + 0.5 * (D)fy);
-    const D jz2 = 2.0 * ((D)m[M04] + 0.5 * (D)fz);
+    const D jx2 = two * ((D)m[M02] + half * (D)fx);
+    const D jy2 = two * ((D)m[M03] + half * (D)fy);
+    const D jz2 = two * ((D)m[M04] + half * (D)fz);
     const D bd = (D)b;
 
     T a[kQ];
@@ -157,16 +161,16 @@ TDLB_HD inline void es_collide(const T* m, T fx, T fy, T fz, T& nuc, const Colli
     a[M08] = (T)((jx2 * (D)uz - (D)(m[M08] * c)) * bd);
     a[M09] = (T)((jy2 * (D)uz - (D)(m[M09] * c)) * bd);
     a[M10] = (T)((jz2 * (D)uz - (D)(m[M10] * c)) * bd);
-    const T ng3 = -P.g3;
+    const T ng3 = -cp.g3;
 #pragma unroll
     for (int l = M11; l <= M16; l++) a[l] = ng3 * m[l];
     a[M17] = T(0.0);
     a[M18] = T(0.0);
 #pragma unroll
-    for (int l = 0; l < kQ; l++) a[l] = div24(a[l]);
+    for (int l = 0; l < kQ; l++) a[l] = Div24<T, D>::apply(a[l]);
 
     const T t2 = T(2), t3 = T(3), t4 = T(4), t6 = T(6);
-    const D o = 1.5;
+    const D o = D(1.5);
     // face populations: integer coefficients, all in T
     p[Q01] = t2 * a[M01] + t4 * a[M02] + t3 * a[M05] - t3 * a[M07] - t3 * a[M10] - t2 * a[M11] - t2 * a[M13]
            + t2 * a[M17] + t2 * a[M18];
@@ -216,5 +220,12 @@ TDLB_HD inline T ib_weight(T xin) {
     }
     return w;
 }
+
+// Reduced-precision population storage: a population is kept as its deviation
+// from the rest population w_d*rho0, scaled by 1/w_d (w = 1/12 on the 6 faces,
+// 1/24 on the 12 edges, the rest populations of this lattice), so the stored
+// value is f/w_d - rho0: O(density and velocity fluctuations), well inside the
+// normal range of fp16 and bf16.  Arithmetic stays in FP32.
+TDLB_HD constexpr float rest_weight_inv(int d) { return d < 6 ? 12.0f : 24.0f; }
 
 }  // namespace tdlb

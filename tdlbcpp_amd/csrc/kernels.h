@@ -1,6 +1,8 @@
 // Host-side launchers of the gfx950 kernels (implemented in kernels.hip).
 // Pointers are device pointers; every launcher enqueues on `s` and returns the
-// launch status.  prec: 0 float, 1 double.  layout: LayoutIJKL/LayoutLIJK.
+// launch status.  prec: 0 float, 1 double, 2 fp16 storage, 3 bf16 storage (the
+// last two compute in float; F, Nu and point arrays are float).
+// layout: LayoutIJKL/LayoutLIJK.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -58,16 +60,25 @@ hipError_t launch_fill_q(int prec, int layout, void* Q, const Grid& g, double rh
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
-hipError_t launch_esotwist(int prec, int layout, int mode, int even, void* Q, const void* F, void* Nu,
-                           const uint64_t* fmask, const Grid& g, const CollideArgs& c, hipStream_t s);
-hipError_t launch_simple(int prec, int layout, int mode, const void* src, void* dst, const void* F, void* Nu,
-                         const uint64_t* fmask, const Grid& g, const CollideArgs& c, hipStream_t s);
+// Populations in Q (stored type), moments in mout (arithmetic type, Q's shape
+// and addressing).  For prec 0/1 mout is Q itself.  For the reduced-precision
+// storage (prec 2 fp16, 3 bf16: populations kept as (f_d - w_d rho0) / w_d)
+// MODE_COLL reads the moments from mout and MODE_MOM / MODE_MOM_PULL write them
+// there; F and Nu are float.
+hipError_t launch_esotwist(int prec, int layout, int mode, int even, void* Q, void* mout, const void* F, void* Nu,
+                           const uint64_t* fmask, const Grid& g, const CollideArgs& c, float rho0, hipStream_t s);
+hipError_t launch_simple(int prec, int layout, int mode, const void* src, void* dst, void* mout, const void* F,
+                         void* Nu, const uint64_t* fmask, const Grid& g, const CollideArgs& c, float rho0,
+                         hipStream_t s);
 
 // element size 4 or 8: pure data movement on every cell
 hipError_t launch_push_all(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s);
 hipError_t launch_pull_all(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s);
 hipError_t launch_bounce_back(int esize, int layout, void* Q, const Grid& g, hipStream_t s);
 hipError_t launch_ghost_fill(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s);
+// ghost fill from reduced-precision populations (prec 2/3) into a float array
+hipError_t launch_ghost_fill_decode(int prec, int layout, const void* src, void* dst, const Grid& g, float rho0,
+                                    hipStream_t s);
 hipError_t launch_ghost_copy(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s);
 
 hipError_t launch_mask_from_f(int prec, const void* F, uint64_t* mask, uint64_t* gmask, const Grid& g,
@@ -79,7 +90,21 @@ hipError_t launch_force_prepare(int prec, const PointsDev& p, int64_t begin, int
                                 const uint64_t* prev, uint64_t* next, const Grid& g, hipStream_t s);
 hipError_t launch_force_level(int prec, int layout, int streaming, int momsrc, int even, const PointsDev& p,
                               int64_t begin, int64_t count, void* F, const void* Q, const Grid& g, double alfa,
-                              double beta, hipStream_t s);
+                              double beta, float rho0, hipStream_t s);
+
+// Density and momentum of IB stencil cells, 4 values of T per entry {rho, jx, jy, jz}.
+// cells: sorted cell indices (one entry per cell).
+hipError_t launch_ib_moments_cells(int prec, int layout, int streaming, int momsrc, int even, const uint32_t* cells,
+                                   int64_t n, void* cache, const void* Q, const Grid& g, float rho0, hipStream_t s);
+// one entry per (point, stencil cell): entry 9*t + s for point begin + t
+hipError_t launch_ib_moments_points(int prec, int layout, int streaming, int momsrc, int even, const PointsDev& p,
+                                    int64_t begin, int64_t count, void* cache, const void* Q, const Grid& g,
+                                    float rho0, hipStream_t s);
+// One forcing level reading stencil moments from a cache.  sidx: per point 9 cache
+// indices (static lists); null: the entries of point begin+t are 9*(t + cache_base) ...
+hipError_t launch_force_level_cached(int prec, const PointsDev& p, int64_t begin, int64_t count,
+                                     const int32_t* sidx, int64_t cache_base, const void* cache, void* F,
+                                     const Grid& g, double alfa, double beta, hipStream_t s);
 hipError_t launch_gen_blades(int prec, const BladeLayout& bl, const BladeStep& st, const PointsDev& out,
                              int64_t offset, hipStream_t s);
 hipError_t launch_solid_mask(double cx, double cz, double radius, uint8_t* out, const Grid& g, int ghost,

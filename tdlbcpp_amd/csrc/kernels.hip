@@ -29,9 +29,44 @@ inline uint32_t blocks_for(uint64_t n, int bs = kBlock) {
     return (uint32_t)(b > 0x7fffffffull ? 0x7fffffffull : (b ? b : 1));
 }
 
-template <typename T>
-__device__ __forceinline__ T ldT(const void* p, uint64_t a) {
-    return static_cast<const T*>(p)[a];
+// ------------------------------------------------------------------ precisions
+// S: stored population type, T: arithmetic and F/Nu type, D: type the
+// reference's double literals promote to.  kShift: populations are stored as
+// scaled deviations from the rest populations (see rest_weight_inv).
+struct PrecF32 {
+    using S = float; using T = float; using D = double;
+    static constexpr bool kShift = false;
+};
+struct PrecF64 {
+    using S = double; using T = double; using D = double;
+    static constexpr bool kShift = false;
+};
+struct PrecF16 {
+    using S = _Float16; using T = float; using D = float;
+    static constexpr bool kShift = true;
+};
+struct PrecBF16 {
+    using S = __bf16; using T = float; using D = float;
+    static constexpr bool kShift = true;
+};
+// bit patterns moved unchanged
+template <typename W>
+struct RawBits {
+    using S = W; using T = W; using D = W;
+    static constexpr bool kShift = false;
+};
+
+template <typename PR>
+__device__ __forceinline__ typename PR::T load_pop(const typename PR::S* __restrict__ Q, uint64_t a, int d,
+                                                   float rho0) {
+    if constexpr (PR::kShift) return ((float)Q[a] + rho0) * (1.0f / rest_weight_inv(d));
+    else return Q[a];
+}
+template <typename PR>
+__device__ __forceinline__ void store_pop(typename PR::S* __restrict__ Q, uint64_t a, int d, typename PR::T v,
+                                          float rho0) {
+    if constexpr (PR::kShift) Q[a] = (typename PR::S)(v * rest_weight_inv(d) - rho0);
+    else Q[a] = v;
 }
 
 // ------------------------------------------------------------------ fills
@@ -68,10 +103,16 @@ __global__ void k_copy_bytes(uint8_t* __restrict__ d, const uint8_t* __restrict_
 // takes the 18 values from there; write puts population d where the read of
 // opp(d) looks (StreamingEsoteric.hpp:82-130); writeMoments puts value d back
 // where d was read.
-template <typename T, int L, int MODE, int EVEN>
-__global__ __launch_bounds__(kBlock) void k_esotwist(T* __restrict__ Q, const T* __restrict__ F,
-                                                     T* __restrict__ Nu, const uint64_t* __restrict__ fmask,
-                                                     Grid g, CollideParams<T> P, uint32_t nkb) {
+// mout receives the moments of MODE_MOM: Q itself (in place) when populations
+// are stored as T, a separate T array in the reference layout otherwise.
+template <typename PR, int L, int MODE, int EVEN>
+__global__ __launch_bounds__(kBlock) void k_esotwist(typename PR::S* __restrict__ Q, typename PR::T* mout,
+                                                     const typename PR::T* __restrict__ F,
+                                                     typename PR::T* __restrict__ Nu,
+                                                     const uint64_t* __restrict__ fmask, Grid g,
+                                                     CollideParams<typename PR::T> P, uint32_t nkb, float rho0) {
+    using T = typename PR::T;
+    using D = typename PR::D;
     const RowBlock rb = row_block(g, blockIdx.x, nkb, blockDim.x);
     const int i = rb.i, j = rb.j, k = rb.k0 + (int)threadIdx.x;
     if (k > g.zg1) return;
@@ -87,13 +128,18 @@ __global__ __launch_bounds__(kBlock) void k_esotwist(T* __restrict__ Q, const T*
 
     T v[kQ];
 #pragma unroll
-    for (int d = 0; d < kQ; d++) v[d] = Q[qaddr<L>(g, home[d], EVEN ? d : opposite(d))];
+    for (int d = 0; d < kQ; d++) {
+        const uint64_t a = qaddr<L>(g, home[d], EVEN ? d : opposite(d));
+        // reduced-precision storage keeps the moments of MODE_COLL in mout
+        if constexpr (PR::kShift && MODE == MODE_COLL) v[d] = mout[a];
+        else v[d] = load_pop<PR>(Q, a, d, rho0);
+    }
 
     if (MODE == MODE_MOM) {
         T m[kQ];
-        es_moments(v, m);
+        es_moments<T, D>(v, m);
 #pragma unroll
-        for (int d = 0; d < kQ; d++) Q[qaddr<L>(g, home[d], EVEN ? d : opposite(d))] = m[d];
+        for (int d = 0; d < kQ; d++) mout[qaddr<L>(g, home[d], EVEN ? d : opposite(d))] = m[d];
         return;
     }
 
@@ -107,16 +153,23 @@ __global__ __launch_bounds__(kBlock) void k_esotwist(T* __restrict__ Q, const T*
     T p[kQ];
     if (MODE == MODE_FUSED) {
         T m[kQ];
-        es_moments(v, m);
-        es_collide(m, fx, fy, fz, nuc, P, p);
+        es_moments<T, D>(v, m);
+        es_collide<T, D>(m, fx, fy, fz, nuc, P, p);
     } else {
-        es_collide(v, fx, fy, fz, nuc, P, p);
+        es_collide<T, D>(v, fx, fy, fz, nuc, P, p);
     }
     if (P.use_les) Nu[n] = nuc;
+    // Recompute the 18 home cells instead of keeping them live through the
+    // collision: the opaque copy stops the compiler from reusing the load
+    // addresses, which frees ~36 VGPRs and raises occupancy.
+    uint32_t n2 = n;
+    int32_t di2 = di, dj2 = dj, dk2 = dk;
+    asm volatile("" : "+v"(n2), "+v"(di2), "+v"(dj2), "+v"(dk2));
 #pragma unroll
     for (int d = 0; d < kQ; d++) {
         const int e = opposite(d);
-        Q[qaddr<L>(g, home[e], EVEN ? e : opposite(e))] = p[d];
+        const uint32_t h = n2 + (uint32_t)((twist_dx(e) ? di2 : 0) + (twist_dy(e) ? dj2 : 0) + (twist_dz(e) ? dk2 : 0));
+        store_pop<PR>(Q, qaddr<L>(g, h, EVEN ? e : opposite(e)), d, p[d], rho0);
     }
 }
 
@@ -124,11 +177,16 @@ __global__ __launch_bounds__(kBlock) void k_esotwist(T* __restrict__ Q, const T*
 // In-place collide / moments, or pull from src (the two-lattice form of the
 // reference's in-place push: slot d of an interior cell c takes src[c-e_d][d]
 // when c-e_d is interior and keeps src[c][d] otherwise).
-template <typename T, int L, int MODE>
-__global__ __launch_bounds__(kBlock) void k_simple(const T* src, T* dst,  // equal for in-place modes
-                                                   const T* __restrict__ F, T* __restrict__ Nu,
+template <typename PR, int L, int MODE>
+__global__ __launch_bounds__(kBlock) void k_simple(const typename PR::S* src,
+                                                   typename PR::S* dst,  // equal to src for in-place modes
+                                                   typename PR::T* mout,  // moments output of the MOM modes
+                                                   const typename PR::T* __restrict__ F,
+                                                   typename PR::T* __restrict__ Nu,
                                                    const uint64_t* __restrict__ fmask, Grid g,
-                                                   CollideParams<T> P, uint32_t nkb) {
+                                                   CollideParams<typename PR::T> P, uint32_t nkb, float rho0) {
+    using T = typename PR::T;
+    using D = typename PR::D;
     const RowBlock rb = row_block(g, blockIdx.x, nkb, blockDim.x);
     const int i = rb.i, j = rb.j, k = rb.k0 + (int)threadIdx.x;
     if (k > g.zg1) return;
@@ -145,18 +203,21 @@ __global__ __launch_bounds__(kBlock) void k_simple(const T* src, T* dst,  // equ
             const bool ok = (ex > 0 ? im : ex < 0 ? ip : true) && (ey > 0 ? jm : ey < 0 ? jp : true) &&
                             (ez > 0 ? km : ez < 0 ? kp : true);
             const int32_t off = -(ex * (int32_t)g.yz + ey * g.zg + ez);
-            v[d] = src[qaddr<L>(g, ok ? n + (uint32_t)off : n, d)];
+            v[d] = load_pop<PR>(src, qaddr<L>(g, ok ? n + (uint32_t)off : n, d), d, rho0);
         }
     } else {
 #pragma unroll
-        for (int d = 0; d < kQ; d++) v[d] = src[qaddr<L>(g, n, d)];
+        for (int d = 0; d < kQ; d++) {
+            if constexpr (PR::kShift && MODE == MODE_COLL) v[d] = mout[qaddr<L>(g, n, d)];
+            else v[d] = load_pop<PR>(src, qaddr<L>(g, n, d), d, rho0);
+        }
     }
 
     if (MODE == MODE_MOM || MODE == MODE_MOM_PULL) {
         T m[kQ];
-        es_moments(v, m);
+        es_moments<T, D>(v, m);
 #pragma unroll
-        for (int d = 0; d < kQ; d++) dst[qaddr<L>(g, n, d)] = m[d];
+        for (int d = 0; d < kQ; d++) mout[qaddr<L>(g, n, d)] = m[d];
         return;
     }
 
@@ -170,14 +231,14 @@ __global__ __launch_bounds__(kBlock) void k_simple(const T* src, T* dst,  // equ
     T p[kQ];
     if (MODE == MODE_FUSED) {
         T m[kQ];
-        es_moments(v, m);
-        es_collide(m, fx, fy, fz, nuc, P, p);
+        es_moments<T, D>(v, m);
+        es_collide<T, D>(m, fx, fy, fz, nuc, P, p);
     } else {
-        es_collide(v, fx, fy, fz, nuc, P, p);
+        es_collide<T, D>(v, fx, fy, fz, nuc, P, p);
     }
     if (P.use_les) Nu[n] = nuc;
 #pragma unroll
-    for (int d = 0; d < kQ; d++) dst[qaddr<L>(g, n, d)] = p[d];
+    for (int d = 0; d < kQ; d++) store_pop<PR>(dst, qaddr<L>(g, n, d), d, p[d], rho0);
 }
 
 // ------------------------------------------------------------------ data movement on every cell
@@ -323,8 +384,9 @@ __host__ __device__ inline uint64_t ghost_count(const Grid& g) {
 // into the ghost layer (c-e_d interior) and the bounce-back slots whose source
 // g+e_d is interior.  Rim bounce-back slots (ghost source) and never-written
 // slots are left as they are in dst.
-template <typename W, int L>
-__global__ void k_ghost_fill(const W* __restrict__ src, W* __restrict__ dst, Grid g) {
+template <typename PR, int L>
+__global__ void k_ghost_fill(const typename PR::S* __restrict__ src, typename PR::T* __restrict__ dst, Grid g,
+                             float rho0) {
     const uint64_t total = ghost_count(g);
     for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total;
          t += (uint64_t)gridDim.x * blockDim.x) {
@@ -340,7 +402,7 @@ __global__ void k_ghost_fill(const W* __restrict__ src, W* __restrict__ dst, Gri
             const int ex = dir_x(d), ey = dir_y(d), ez = dir_z(d);
             const int si = i - ex, sj = j - ey, sk = k - ez;
             if (is_interior(g, si, sj, sk)) {
-                dst[qaddr<L>(g, c, d)] = src[qaddr<L>(g, cell_index(g, si, sj, sk), d)];
+                dst[qaddr<L>(g, c, d)] = load_pop<PR>(src, qaddr<L>(g, cell_index(g, si, sj, sk), d), d, rho0);
                 continue;
             }
             bool bb = false;
@@ -353,7 +415,8 @@ __global__ void k_ghost_fill(const W* __restrict__ src, W* __restrict__ dst, Gri
             if (!bb) continue;
             const int ti = i + ex, tj = j + ey, tk = k + ez;
             if (!is_interior(g, ti, tj, tk)) continue;
-            dst[qaddr<L>(g, c, d)] = src[qaddr<L>(g, cell_index(g, ti, tj, tk), opposite(d))];
+            // opp(d) has the rest weight of d
+            dst[qaddr<L>(g, c, d)] = load_pop<PR>(src, qaddr<L>(g, cell_index(g, ti, tj, tk), opposite(d)), d, rho0);
         }
     }
 }
@@ -454,9 +517,11 @@ __global__ void k_force_prepare(PointsDev p, int64_t begin, int64_t count, T* __
 }
 
 // density and momentum of cell (i,j,k) as the reference's moments() would leave them
-template <typename T, int L, int STREAM, int MOMSRC, int EVEN>
-__device__ __forceinline__ void cell_rho_j(const T* __restrict__ Q, const Grid& g, int i, int j, int k, T& rho,
-                                           T& jx, T& jy, T& jz) {
+template <typename PR, int L, int STREAM, int MOMSRC, int EVEN>
+__device__ __forceinline__ void cell_rho_j(const typename PR::S* __restrict__ Q, const Grid& g, int i, int j, int k,
+                                           typename PR::T& rho, typename PR::T& jx, typename PR::T& jy,
+                                           typename PR::T& jz, float rho0) {
+    using T = typename PR::T;
     const uint32_t n = cell_index(g, i, j, k);
     if (STREAM == StreamEsotwist) {
         const int32_t di = (i == g.xg1) ? -(int32_t)((uint32_t)(g.xg1 - 1) * g.yz) : (int32_t)g.yz;
@@ -467,7 +532,7 @@ __device__ __forceinline__ void cell_rho_j(const T* __restrict__ Q, const Grid& 
 #pragma unroll
             for (int d = 0; d < 4; d++) {
                 const uint32_t h = n + (uint32_t)((twist_dx(d) ? di : 0) + (twist_dy(d) ? dj : 0) + (twist_dz(d) ? dk : 0));
-                m[d] = Q[qaddr<L>(g, h, EVEN ? d : opposite(d))];
+                m[d] = (T)Q[qaddr<L>(g, h, EVEN ? d : opposite(d))];
             }
             rho = m[0]; jx = m[1]; jy = m[2]; jz = m[3];
         } else {
@@ -475,16 +540,16 @@ __device__ __forceinline__ void cell_rho_j(const T* __restrict__ Q, const Grid& 
 #pragma unroll
             for (int d = 0; d < kQ; d++) {
                 const uint32_t h = n + (uint32_t)((twist_dx(d) ? di : 0) + (twist_dy(d) ? dj : 0) + (twist_dz(d) ? dk : 0));
-                f[d] = Q[qaddr<L>(g, h, EVEN ? d : opposite(d))];
+                f[d] = load_pop<PR>(Q, qaddr<L>(g, h, EVEN ? d : opposite(d)), d, rho0);
             }
             es_rho_j(f, rho, jx, jy, jz);
         }
     } else {
         if (MOMSRC == MOMSRC_STORED) {
-            rho = Q[qaddr<L>(g, n, 0)];
-            jx = Q[qaddr<L>(g, n, 1)];
-            jy = Q[qaddr<L>(g, n, 2)];
-            jz = Q[qaddr<L>(g, n, 3)];
+            rho = (T)Q[qaddr<L>(g, n, 0)];
+            jx = (T)Q[qaddr<L>(g, n, 1)];
+            jy = (T)Q[qaddr<L>(g, n, 2)];
+            jz = (T)Q[qaddr<L>(g, n, 3)];
         } else {
             const bool im = i >= 2, ip = i <= g.xg1 - 1;
             const bool jm = j >= 2, jp = j <= g.yg1 - 1;
@@ -496,7 +561,7 @@ __device__ __forceinline__ void cell_rho_j(const T* __restrict__ Q, const Grid& 
                 const bool ok = (ex > 0 ? im : ex < 0 ? ip : true) && (ey > 0 ? jm : ey < 0 ? jp : true) &&
                                 (ez > 0 ? km : ez < 0 ? kp : true);
                 const int32_t off = -(ex * (int32_t)g.yz + ey * g.zg + ez);
-                f[d] = Q[qaddr<L>(g, ok ? n + (uint32_t)off : n, d)];
+                f[d] = load_pop<PR>(Q, qaddr<L>(g, ok ? n + (uint32_t)off : n, d), d, rho0);
             }
             es_rho_j(f, rho, jx, jy, jz);
         }
@@ -506,10 +571,13 @@ __device__ __forceinline__ void cell_rho_j(const T* __restrict__ Q, const Grid& 
 // One level of the forcing: points of a level have pairwise disjoint stencils,
 // so each thread runs the reference's gather / correct / spread for one point
 // (Forcing.hpp:73-135) with the F values left by all earlier levels.
-template <typename T, int L, int STREAM, int MOMSRC, int EVEN>
+template <typename PR, int L, int STREAM, int MOMSRC, int EVEN>
 __global__ __launch_bounds__(kBlock) void k_force_level(PointsDev p, int64_t begin, int64_t count,
-                                                        T* __restrict__ F, const T* __restrict__ Q, Grid g,
-                                                        T alfa, T beta) {
+                                                        typename PR::T* __restrict__ F,
+                                                        const typename PR::S* __restrict__ Q, Grid g,
+                                                        typename PR::T alfa, typename PR::T beta, float rho0) {
+    using T = typename PR::T;
+    using D = typename PR::D;
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= count) return;
     const int64_t q = begin + t;
@@ -528,12 +596,12 @@ __global__ __launch_bounds__(kBlock) void k_force_level(PointsDev p, int64_t beg
         for (int i1 = -1; i1 <= 1; i1++) {
             const int i2 = wrap_ik(pi + i1, g.xg1), k2 = wrap_ik(pk + k1, g.zg1);
             T rho, jx, jy, jz;
-            cell_rho_j<T, L, STREAM, MOMSRC, EVEN>(Q, g, i2, pj, k2, rho, jx, jy, jz);
+            cell_rho_j<PR, L, STREAM, MOMSRC, EVEN>(Q, g, i2, pj, k2, rho, jx, jy, jz, rho0);
             const uint64_t c = cell_index(g, i2, pj, k2);
             const T ppp = wx[i1 + 1] * wz[k1 + 1];
-            const T xx = (T)((double)jx + 0.5 * (double)F[3 * c + 0]);
-            const T yy = (T)((double)jy + 0.5 * (double)F[3 * c + 1]);
-            const T zz = (T)((double)jz + 0.5 * (double)F[3 * c + 2]);
+            const T xx = (T)((D)jx + D(0.5) * (D)F[3 * c + 0]);
+            const T yy = (T)((D)jy + D(0.5) * (D)F[3 * c + 1]);
+            const T zz = (T)((D)jz + D(0.5) * (D)F[3 * c + 2]);
             rhoSum += ppp * rho;
             xSum += ppp * xx;
             ySum += ppp * yy;
@@ -553,6 +621,105 @@ __global__ __launch_bounds__(kBlock) void k_force_level(PointsDev p, int64_t beg
             F[3 * c + 1] = alfa * F[3 * c + 1] - bp * ySum;
             F[3 * c + 2] = alfa * F[3 * c + 2] - bp * zSum;
         }
+}
+
+// ------------------------------------------------------------------ IB forcing with a moments cache
+// Density and momentum of the stencil cells are computed once per forcing pass
+// (they do not change while F is updated), so the level kernels issue only
+// independent cache / F loads instead of 18 scattered population loads per
+// stencil cell.
+template <typename T>
+struct alignas(4 * sizeof(T)) Mom4 {
+    T r, x, y, z;
+};
+
+template <typename PR, int L, int STREAM, int MOMSRC, int EVEN>
+__global__ __launch_bounds__(kBlock) void k_ib_moments_cells(const uint32_t* __restrict__ cells, int64_t n,
+                                                             Mom4<typename PR::T>* __restrict__ cache,
+                                                             const typename PR::S* __restrict__ Q, Grid g,
+                                                             float rho0) {
+    using T = typename PR::T;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t c = cells[t];
+    const int i = (int)(c / g.yz);
+    const uint32_t r = c - (uint32_t)i * g.yz;
+    const int j = (int)(r / (uint32_t)g.zg);
+    const int k = (int)(r - (uint32_t)j * (uint32_t)g.zg);
+    Mom4<T> m;
+    cell_rho_j<PR, L, STREAM, MOMSRC, EVEN>(Q, g, i, j, k, m.r, m.x, m.y, m.z, rho0);
+    cache[t] = m;
+}
+
+template <typename PR, int L, int STREAM, int MOMSRC, int EVEN>
+__global__ __launch_bounds__(kBlock) void k_ib_moments_points(PointsDev p, int64_t begin, int64_t count,
+                                                              Mom4<typename PR::T>* __restrict__ cache,
+                                                              const typename PR::S* __restrict__ Q, Grid g,
+                                                              float rho0) {
+    using T = typename PR::T;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= count * 9) return;
+    const int64_t q = begin + t / 9;
+    const int s = (int)(t % 9);
+    const int i2 = wrap_ik(p.i[q] + (s % 3) - 1, g.xg1), k2 = wrap_ik(p.k[q] + (s / 3) - 1, g.zg1);
+    Mom4<T> m;
+    cell_rho_j<PR, L, STREAM, MOMSRC, EVEN>(Q, g, i2, p.j[q], k2, m.r, m.x, m.y, m.z, rho0);
+    cache[t] = m;
+}
+
+// Same arithmetic as k_force_level; stencil cell s = (k1+1)*3 + (i1+1) reads
+// cache[sidx[9q+s]] (INDEXED) or cache[9(q-cache_base)+s].
+template <typename T, typename D, bool INDEXED>
+__global__ __launch_bounds__(kBlock) void k_force_level_cached(PointsDev p, int64_t begin, int64_t count,
+                                                               const int32_t* __restrict__ sidx, int64_t cache_base,
+                                                               const Mom4<T>* __restrict__ cache, T* __restrict__ F,
+                                                               Grid g, T alfa, T beta) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int64_t q = begin + t;
+    const int pi = p.i[q], pj = p.j[q], pk = p.k[q];
+    const T ifr = static_cast<const T*>(p.ifrac)[q], kfr = static_cast<const T*>(p.kfrac)[q];
+    T wx[3], wz[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        wx[a] = ib_weight<T>(-ifr + T(a - 1));
+        wz[a] = ib_weight<T>(-kfr + T(a - 1));
+    }
+    uint64_t cell[9];
+    Mom4<T> mom[9];
+    T fo[9][3];
+#pragma unroll
+    for (int s = 0; s < 9; s++) {
+        const int i2 = wrap_ik(pi + (s % 3) - 1, g.xg1), k2 = wrap_ik(pk + (s / 3) - 1, g.zg1);
+        cell[s] = cell_index(g, i2, pj, k2);
+        const int64_t e = INDEXED ? (int64_t)sidx[9 * q + s] : 9 * (q - cache_base) + s;
+        mom[s] = cache[e];
+        fo[s][0] = F[3 * cell[s] + 0];
+        fo[s][1] = F[3 * cell[s] + 1];
+        fo[s][2] = F[3 * cell[s] + 2];
+    }
+    T rhoSum = T(0.0), xSum = T(0.0), ySum = T(0.0), zSum = T(0.0);
+#pragma unroll
+    for (int s = 0; s < 9; s++) {
+        const T ppp = wx[s % 3] * wz[s / 3];
+        const T xx = (T)((D)mom[s].x + D(0.5) * (D)fo[s][0]);
+        const T yy = (T)((D)mom[s].y + D(0.5) * (D)fo[s][1]);
+        const T zz = (T)((D)mom[s].z + D(0.5) * (D)fo[s][2]);
+        rhoSum += ppp * mom[s].r;
+        xSum += ppp * xx;
+        ySum += ppp * yy;
+        zSum += ppp * zz;
+    }
+    xSum -= rhoSum * static_cast<const T*>(p.u)[q];
+    ySum -= rhoSum * static_cast<const T*>(p.v)[q];
+    zSum -= rhoSum * static_cast<const T*>(p.w)[q];
+#pragma unroll
+    for (int s = 0; s < 9; s++) {
+        const T bp = beta * (wx[s % 3] * wz[s / 3]);
+        F[3 * cell[s] + 0] = alfa * fo[s][0] - bp * xSum;
+        F[3 * cell[s] + 1] = alfa * fo[s][1] - bp * ySum;
+        F[3 * cell[s] + 2] = alfa * fo[s][2] - bp * zSum;
+    }
 }
 
 // ------------------------------------------------------------------ geometry on the device
@@ -596,6 +763,10 @@ template <typename T>
 CollideParams<T> to_params(const CollideArgs& c) {
     return make_collide_params<T>(T(c.nu), T(c.cs0), T(c.g3), c.use_les);
 }
+template <typename T, typename D>
+CollideParams<T> to_params(const CollideArgs& c) {
+    return make_collide_params<T, D>(T(c.nu), T(c.cs0), T(c.g3), c.use_les);
+}
 
 uint32_t grid_blocks(uint64_t n) {
     uint64_t b = (n + kBlock - 1) / kBlock;
@@ -609,15 +780,20 @@ inline void row_launch(const Grid& g, uint32_t& nkb, uint32_t& nblocks) {
     nblocks = nkb * (uint32_t)g.yg1 * (uint32_t)g.xg1;
 }
 
-template <typename T, int L>
-hipError_t esotwist_t(int mode, int even, void* Q, const void* F, void* Nu, const uint64_t* fm, const Grid& g,
-                      const CollideParams<T>& P, hipStream_t s) {
+template <typename PR, int L>
+hipError_t esotwist_t(int mode, int even, void* Q, void* mout, const void* F, void* Nu, const uint64_t* fm,
+                      const Grid& g, const CollideArgs& c, float rho0, hipStream_t s) {
+    using S = typename PR::S;
+    using T = typename PR::T;
     uint32_t nkb, nb;
     row_launch(g, nkb, nb);
-    T* q = static_cast<T*>(Q);
+    S* q = static_cast<S*>(Q);
+    T* mo = static_cast<T*>(mout);
     const T* f = static_cast<const T*>(F);
     T* nu = static_cast<T*>(Nu);
-#define TDLB_ESO(M, E) hipLaunchKernelGGL((k_esotwist<T, L, M, E>), dim3(nb), dim3(kBlock), 0, s, q, f, nu, fm, g, P, nkb)
+    const auto P = to_params<T, typename PR::D>(c);
+#define TDLB_ESO(M, E) \
+    hipLaunchKernelGGL((k_esotwist<PR, L, M, E>), dim3(nb), dim3(kBlock), 0, s, q, mo, f, nu, fm, g, P, nkb, rho0)
     if (mode == MODE_COLL) { if (even) TDLB_ESO(MODE_COLL, 1); else TDLB_ESO(MODE_COLL, 0); }
     else if (mode == MODE_MOM) { if (even) TDLB_ESO(MODE_MOM, 1); else TDLB_ESO(MODE_MOM, 0); }
     else if (mode == MODE_FUSED) { if (even) TDLB_ESO(MODE_FUSED, 1); else TDLB_ESO(MODE_FUSED, 0); }
@@ -626,16 +802,21 @@ hipError_t esotwist_t(int mode, int even, void* Q, const void* F, void* Nu, cons
     return hipGetLastError();
 }
 
-template <typename T, int L>
-hipError_t simple_t(int mode, const void* src, void* dst, const void* F, void* Nu, const uint64_t* fm,
-                    const Grid& g, const CollideParams<T>& P, hipStream_t s) {
+template <typename PR, int L>
+hipError_t simple_t(int mode, const void* src, void* dst, void* mout, const void* F, void* Nu, const uint64_t* fm,
+                    const Grid& g, const CollideArgs& c, float rho0, hipStream_t s) {
+    using S = typename PR::S;
+    using T = typename PR::T;
     uint32_t nkb, nb;
     row_launch(g, nkb, nb);
-    const T* a = static_cast<const T*>(src);
-    T* b = static_cast<T*>(dst);
+    const S* a = static_cast<const S*>(src);
+    S* b = static_cast<S*>(dst);
+    T* mo = static_cast<T*>(mout);
     const T* f = static_cast<const T*>(F);
     T* nu = static_cast<T*>(Nu);
-#define TDLB_SIM(M) hipLaunchKernelGGL((k_simple<T, L, M>), dim3(nb), dim3(kBlock), 0, s, a, b, f, nu, fm, g, P, nkb)
+    const auto P = to_params<T, typename PR::D>(c);
+#define TDLB_SIM(M) \
+    hipLaunchKernelGGL((k_simple<PR, L, M>), dim3(nb), dim3(kBlock), 0, s, a, b, mo, f, nu, fm, g, P, nkb, rho0)
     switch (mode) {
         case MODE_COLL: TDLB_SIM(MODE_COLL); break;
         case MODE_MOM: TDLB_SIM(MODE_MOM); break;
@@ -647,15 +828,21 @@ hipError_t simple_t(int mode, const void* src, void* dst, const void* F, void* N
     return hipGetLastError();
 }
 
-template <typename T, int L, int STREAM>
+template <typename PR, int L, int STREAM>
 hipError_t force_level_t(int momsrc, int even, const PointsDev& p, int64_t begin, int64_t count, void* F,
-                         const void* Q, const Grid& g, T alfa, T beta, hipStream_t s) {
+                         const void* Q, const Grid& g, double alfa, double beta, float rho0, hipStream_t s) {
+    using S = typename PR::S;
+    using T = typename PR::T;
     const uint32_t nb = (uint32_t)((count + kBlock - 1) / kBlock);
     T* f = static_cast<T*>(F);
-    const T* q = static_cast<const T*>(Q);
+    const S* q = static_cast<const S*>(Q);
+    const T a = (T)alfa, b = (T)beta;
 #define TDLB_FL(M, E) \
-    hipLaunchKernelGGL((k_force_level<T, L, STREAM, M, E>), dim3(nb), dim3(kBlock), 0, s, p, begin, count, f, q, g, alfa, beta)
-    if (momsrc == MOMSRC_STORED) { if (even) TDLB_FL(MOMSRC_STORED, 1); else TDLB_FL(MOMSRC_STORED, 0); }
+    hipLaunchKernelGGL((k_force_level<PR, L, STREAM, M, E>), dim3(nb), dim3(kBlock), 0, s, p, begin, count, f, q, g, a, b, rho0)
+    if (momsrc == MOMSRC_STORED) {
+        if constexpr (PR::kShift) return hipErrorInvalidValue;
+        else { if (even) TDLB_FL(MOMSRC_STORED, 1); else TDLB_FL(MOMSRC_STORED, 0); }
+    }
     else if (momsrc == MOMSRC_ESO_POPS) { if (even) TDLB_FL(MOMSRC_ESO_POPS, 1); else TDLB_FL(MOMSRC_ESO_POPS, 0); }
     else { TDLB_FL(MOMSRC_PULL_POPS, 1); }
 #undef TDLB_FL
@@ -663,6 +850,17 @@ hipError_t force_level_t(int momsrc, int even, const PointsDev& p, int64_t begin
 }
 
 }  // namespace
+
+// precision index -> traits: 0 fp32, 1 fp64, 2 fp16 storage, 3 bf16 storage
+#define TDLB_WITH_PREC(prec, ...)                      \
+    do {                                               \
+        switch (prec) {                                \
+            case 0: { using PR = PrecF32; __VA_ARGS__; } break;  \
+            case 1: { using PR = PrecF64; __VA_ARGS__; } break;  \
+            case 2: { using PR = PrecF16; __VA_ARGS__; } break;  \
+            default: { using PR = PrecBF16; __VA_ARGS__; } break; \
+        }                                              \
+    } while (0)
 
 // ====================================================================== launchers
 hipError_t launch_fill_q(int prec, int layout, void* Q, const Grid& g, double rho, hipStream_t s) {
@@ -697,28 +895,23 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) 
     return hipGetLastError();
 }
 
-hipError_t launch_esotwist(int prec, int layout, int mode, int even, void* Q, const void* F, void* Nu,
-                           const uint64_t* fmask, const Grid& g, const CollideArgs& c, hipStream_t s) {
-    if (prec == 0) {
-        const auto P = to_params<float>(c);
-        return layout == LayoutIJKL ? esotwist_t<float, LayoutIJKL>(mode, even, Q, F, Nu, fmask, g, P, s)
-                                    : esotwist_t<float, LayoutLIJK>(mode, even, Q, F, Nu, fmask, g, P, s);
-    }
-    const auto P = to_params<double>(c);
-    return layout == LayoutIJKL ? esotwist_t<double, LayoutIJKL>(mode, even, Q, F, Nu, fmask, g, P, s)
-                                : esotwist_t<double, LayoutLIJK>(mode, even, Q, F, Nu, fmask, g, P, s);
+hipError_t launch_esotwist(int prec, int layout, int mode, int even, void* Q, void* mout, const void* F, void* Nu,
+                           const uint64_t* fmask, const Grid& g, const CollideArgs& c, float rho0, hipStream_t s) {
+    hipError_t e = hipErrorInvalidValue;
+    TDLB_WITH_PREC(prec, e = layout == LayoutIJKL
+                                 ? esotwist_t<PR, LayoutIJKL>(mode, even, Q, mout, F, Nu, fmask, g, c, rho0, s)
+                                 : esotwist_t<PR, LayoutLIJK>(mode, even, Q, mout, F, Nu, fmask, g, c, rho0, s));
+    return e;
 }
 
-hipError_t launch_simple(int prec, int layout, int mode, const void* src, void* dst, const void* F, void* Nu,
-                         const uint64_t* fmask, const Grid& g, const CollideArgs& c, hipStream_t s) {
-    if (prec == 0) {
-        const auto P = to_params<float>(c);
-        return layout == LayoutIJKL ? simple_t<float, LayoutIJKL>(mode, src, dst, F, Nu, fmask, g, P, s)
-                                    : simple_t<float, LayoutLIJK>(mode, src, dst, F, Nu, fmask, g, P, s);
-    }
-    const auto P = to_params<double>(c);
-    return layout == LayoutIJKL ? simple_t<double, LayoutIJKL>(mode, src, dst, F, Nu, fmask, g, P, s)
-                                : simple_t<double, LayoutLIJK>(mode, src, dst, F, Nu, fmask, g, P, s);
+hipError_t launch_simple(int prec, int layout, int mode, const void* src, void* dst, void* mout, const void* F,
+                         void* Nu, const uint64_t* fmask, const Grid& g, const CollideArgs& c, float rho0,
+                         hipStream_t s) {
+    hipError_t e = hipErrorInvalidValue;
+    TDLB_WITH_PREC(prec, e = layout == LayoutIJKL
+                                 ? simple_t<PR, LayoutIJKL>(mode, src, dst, mout, F, Nu, fmask, g, c, rho0, s)
+                                 : simple_t<PR, LayoutLIJK>(mode, src, dst, mout, F, Nu, fmask, g, c, rho0, s));
+    return e;
 }
 
 // launch KERNEL<W, layout> over NTHREADS grid-stride threads
@@ -759,8 +952,21 @@ hipError_t launch_bounce_back(int esize, int layout, void* Q, const Grid& g, hip
 }
 
 hipError_t launch_ghost_fill(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s) {
-    if (esize == 4) TDLB_MOVE(k_ghost_fill, uint32_t, ghost_count(g), (const uint32_t*)src, (uint32_t*)dst, g);
-    else TDLB_MOVE(k_ghost_fill, uint64_t, ghost_count(g), (const uint64_t*)src, (uint64_t*)dst, g);
+    if (esize == 4)
+        TDLB_MOVE(k_ghost_fill, RawBits<uint32_t>, ghost_count(g), (const uint32_t*)src, (uint32_t*)dst, g, 0.0f);
+    else
+        TDLB_MOVE(k_ghost_fill, RawBits<uint64_t>, ghost_count(g), (const uint64_t*)src, (uint64_t*)dst, g, 0.0f);
+    return hipGetLastError();
+}
+
+hipError_t launch_ghost_fill_decode(int prec, int layout, const void* src, void* dst, const Grid& g, float rho0,
+                                    hipStream_t s) {
+    if (prec == 2)
+        TDLB_MOVE(k_ghost_fill, PrecF16, ghost_count(g), (const _Float16*)src, (float*)dst, g, rho0);
+    else if (prec == 3)
+        TDLB_MOVE(k_ghost_fill, PrecBF16, ghost_count(g), (const __bf16*)src, (float*)dst, g, rho0);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -774,13 +980,13 @@ hipError_t launch_ghost_copy(int esize, int layout, const void* src, void* dst, 
 hipError_t launch_mask_from_f(int prec, const void* F, uint64_t* mask, uint64_t* gmask, const Grid& g,
                               hipStream_t s) {
     const uint64_t words = (uint64_t)g.xg * g.yg * g.mask_w;
-    if (prec == 0) hipLaunchKernelGGL(k_mask_from_f<float>, dim3(grid_blocks(words)), dim3(kBlock), 0, s, (const float*)F, mask, gmask, g);
+    if (prec != 1) hipLaunchKernelGGL(k_mask_from_f<float>, dim3(grid_blocks(words)), dim3(kBlock), 0, s, (const float*)F, mask, gmask, g);
     else hipLaunchKernelGGL(k_mask_from_f<double>, dim3(grid_blocks(words)), dim3(kBlock), 0, s, (const double*)F, mask, gmask, g);
     return hipGetLastError();
 }
 
 hipError_t launch_masked_f(int prec, const void* F, const uint64_t* mask, void* out, const Grid& g, hipStream_t s) {
-    if (prec == 0) hipLaunchKernelGGL(k_masked_f<float>, dim3(grid_blocks(g.size)), dim3(kBlock), 0, s, (const float*)F, mask, (float*)out, g);
+    if (prec != 1) hipLaunchKernelGGL(k_masked_f<float>, dim3(grid_blocks(g.size)), dim3(kBlock), 0, s, (const float*)F, mask, (float*)out, g);
     else hipLaunchKernelGGL(k_masked_f<double>, dim3(grid_blocks(g.size)), dim3(kBlock), 0, s, (const double*)F, mask, (double*)out, g);
     return hipGetLastError();
 }
@@ -794,34 +1000,105 @@ hipError_t launch_force_prepare(int prec, const PointsDev& p, int64_t begin, int
                                 const uint64_t* prev, uint64_t* next, const Grid& g, hipStream_t s) {
     if (count <= 0) return hipSuccess;
     const uint32_t nb = (uint32_t)((count + kBlock - 1) / kBlock);
-    if (prec == 0) hipLaunchKernelGGL(k_force_prepare<float>, dim3(nb), dim3(kBlock), 0, s, p, begin, count, (float*)F, prev, next, g);
+    if (prec != 1) hipLaunchKernelGGL(k_force_prepare<float>, dim3(nb), dim3(kBlock), 0, s, p, begin, count, (float*)F, prev, next, g);
     else hipLaunchKernelGGL(k_force_prepare<double>, dim3(nb), dim3(kBlock), 0, s, p, begin, count, (double*)F, prev, next, g);
     return hipGetLastError();
 }
 
 hipError_t launch_force_level(int prec, int layout, int streaming, int momsrc, int even, const PointsDev& p,
                               int64_t begin, int64_t count, void* F, const void* Q, const Grid& g, double alfa,
-                              double beta, hipStream_t s) {
+                              double beta, float rho0, hipStream_t s) {
     if (count <= 0) return hipSuccess;
-    if (prec == 0) {
-        const float a = (float)alfa, b = (float)beta;
-        if (layout == LayoutIJKL) {
-            return streaming == StreamEsotwist
-                       ? force_level_t<float, LayoutIJKL, StreamEsotwist>(momsrc, even, p, begin, count, F, Q, g, a, b, s)
-                       : force_level_t<float, LayoutIJKL, StreamSimple>(momsrc, even, p, begin, count, F, Q, g, a, b, s);
-        }
-        return streaming == StreamEsotwist
-                   ? force_level_t<float, LayoutLIJK, StreamEsotwist>(momsrc, even, p, begin, count, F, Q, g, a, b, s)
-                   : force_level_t<float, LayoutLIJK, StreamSimple>(momsrc, even, p, begin, count, F, Q, g, a, b, s);
+    hipError_t e = hipErrorInvalidValue;
+    TDLB_WITH_PREC(prec, {
+        if (layout == LayoutIJKL)
+            e = streaming == StreamEsotwist
+                    ? force_level_t<PR, LayoutIJKL, StreamEsotwist>(momsrc, even, p, begin, count, F, Q, g, alfa, beta, rho0, s)
+                    : force_level_t<PR, LayoutIJKL, StreamSimple>(momsrc, even, p, begin, count, F, Q, g, alfa, beta, rho0, s);
+        else
+            e = streaming == StreamEsotwist
+                    ? force_level_t<PR, LayoutLIJK, StreamEsotwist>(momsrc, even, p, begin, count, F, Q, g, alfa, beta, rho0, s)
+                    : force_level_t<PR, LayoutLIJK, StreamSimple>(momsrc, even, p, begin, count, F, Q, g, alfa, beta, rho0, s);
+    });
+    return e;
+}
+
+namespace {
+template <typename PR, int L, int STREAM>
+hipError_t ib_moments_t(int momsrc, int even, const uint32_t* cells, const PointsDev* p, int64_t begin, int64_t n,
+                        void* cache, const void* Q, const Grid& g, float rho0, hipStream_t s) {
+    using T = typename PR::T;
+    using S = typename PR::S;
+    Mom4<T>* c = static_cast<Mom4<T>*>(cache);
+    const S* q = static_cast<const S*>(Q);
+    const int64_t threads = cells ? n : 9 * n;
+    const uint32_t nb = (uint32_t)((threads + kBlock - 1) / kBlock);
+#define TDLB_IBM(M, E)                                                                                          \
+    do {                                                                                                        \
+        if (cells) hipLaunchKernelGGL((k_ib_moments_cells<PR, L, STREAM, M, E>), dim3(nb), dim3(kBlock), 0, s, cells, n, c, q, g, rho0); \
+        else hipLaunchKernelGGL((k_ib_moments_points<PR, L, STREAM, M, E>), dim3(nb), dim3(kBlock), 0, s, *p, begin, n, c, q, g, rho0); \
+    } while (0)
+    if (momsrc == MOMSRC_STORED) {
+        if constexpr (PR::kShift) return hipErrorInvalidValue;
+        else { if (even) TDLB_IBM(MOMSRC_STORED, 1); else TDLB_IBM(MOMSRC_STORED, 0); }
     }
-    if (layout == LayoutIJKL) {
+    else if (momsrc == MOMSRC_ESO_POPS) { if (even) TDLB_IBM(MOMSRC_ESO_POPS, 1); else TDLB_IBM(MOMSRC_ESO_POPS, 0); }
+    else TDLB_IBM(MOMSRC_PULL_POPS, 1);
+#undef TDLB_IBM
+    return hipGetLastError();
+}
+
+template <typename PR>
+hipError_t ib_moments_prec(int layout, int streaming, int momsrc, int even, const uint32_t* cells, const PointsDev* p,
+                           int64_t begin, int64_t n, void* cache, const void* Q, const Grid& g, float rho0,
+                           hipStream_t s) {
+    if (layout == LayoutIJKL)
         return streaming == StreamEsotwist
-                   ? force_level_t<double, LayoutIJKL, StreamEsotwist>(momsrc, even, p, begin, count, F, Q, g, alfa, beta, s)
-                   : force_level_t<double, LayoutIJKL, StreamSimple>(momsrc, even, p, begin, count, F, Q, g, alfa, beta, s);
-    }
+                   ? ib_moments_t<PR, LayoutIJKL, StreamEsotwist>(momsrc, even, cells, p, begin, n, cache, Q, g, rho0, s)
+                   : ib_moments_t<PR, LayoutIJKL, StreamSimple>(momsrc, even, cells, p, begin, n, cache, Q, g, rho0, s);
     return streaming == StreamEsotwist
-               ? force_level_t<double, LayoutLIJK, StreamEsotwist>(momsrc, even, p, begin, count, F, Q, g, alfa, beta, s)
-               : force_level_t<double, LayoutLIJK, StreamSimple>(momsrc, even, p, begin, count, F, Q, g, alfa, beta, s);
+               ? ib_moments_t<PR, LayoutLIJK, StreamEsotwist>(momsrc, even, cells, p, begin, n, cache, Q, g, rho0, s)
+               : ib_moments_t<PR, LayoutLIJK, StreamSimple>(momsrc, even, cells, p, begin, n, cache, Q, g, rho0, s);
+}
+}  // namespace
+
+hipError_t launch_ib_moments_cells(int prec, int layout, int streaming, int momsrc, int even, const uint32_t* cells,
+                                   int64_t n, void* cache, const void* Q, const Grid& g, float rho0, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipError_t e = hipErrorInvalidValue;
+    TDLB_WITH_PREC(prec, e = ib_moments_prec<PR>(layout, streaming, momsrc, even, cells, nullptr, 0, n, cache, Q, g,
+                                                 rho0, s));
+    return e;
+}
+
+hipError_t launch_ib_moments_points(int prec, int layout, int streaming, int momsrc, int even, const PointsDev& p,
+                                    int64_t begin, int64_t count, void* cache, const void* Q, const Grid& g,
+                                    float rho0, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipError_t e = hipErrorInvalidValue;
+    TDLB_WITH_PREC(prec, e = ib_moments_prec<PR>(layout, streaming, momsrc, even, nullptr, &p, begin, count, cache, Q,
+                                                 g, rho0, s));
+    return e;
+}
+
+hipError_t launch_force_level_cached(int prec, const PointsDev& p, int64_t begin, int64_t count,
+                                     const int32_t* sidx, int64_t cache_base, const void* cache, void* F,
+                                     const Grid& g, double alfa, double beta, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    constexpr int bs = 64;  // levels are small: more blocks spread over more CUs
+    const uint32_t nb = (uint32_t)((count + bs - 1) / bs);
+    TDLB_WITH_PREC(prec, {
+        using T = typename PR::T;
+        using D = typename PR::D;
+        const auto* c = static_cast<const Mom4<T>*>(cache);
+        if (sidx)
+            hipLaunchKernelGGL((k_force_level_cached<T, D, true>), dim3(nb), dim3(bs), 0, s, p, begin, count, sidx,
+                               cache_base, c, (T*)F, g, (T)alfa, (T)beta);
+        else
+            hipLaunchKernelGGL((k_force_level_cached<T, D, false>), dim3(nb), dim3(bs), 0, s, p, begin, count, sidx,
+                               cache_base, c, (T*)F, g, (T)alfa, (T)beta);
+    });
+    return hipGetLastError();
 }
 
 hipError_t launch_gen_blades(int prec, const BladeLayout& bl, const BladeStep& st, const PointsDev& out,
@@ -829,7 +1106,7 @@ hipError_t launch_gen_blades(int prec, const BladeLayout& bl, const BladeStep& s
     const int64_t count = (int64_t)bl.n_blades * bl.n_planes * bl.n_radial;
     if (count <= 0) return hipSuccess;
     const uint32_t nb = (uint32_t)((count + kBlock - 1) / kBlock);
-    if (prec == 0) hipLaunchKernelGGL(k_gen_blades<float>, dim3(nb), dim3(kBlock), 0, s, bl, st, out, offset, count);
+    if (prec != 1) hipLaunchKernelGGL(k_gen_blades<float>, dim3(nb), dim3(kBlock), 0, s, bl, st, out, offset, count);
     else hipLaunchKernelGGL(k_gen_blades<double>, dim3(nb), dim3(kBlock), 0, s, bl, st, out, offset, count);
     return hipGetLastError();
 }

@@ -1,0 +1,118 @@
+"""Checkpoint files and the C++ driver on the GPU.
+
+The reference's own checkpoint test (ComputeUnitCheckpointTest.cpp:46-76) is a
+write-then-read round trip of every field; this file does that through the C
+ABI, checks the files hold the raw arrays in the reference's layout, and runs
+the driver end to end on configuration 1 (60^3 Rushton tank, Simple streaming,
+IJKL, FP32, LES) against the oracle.
+"""
+import glob
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import tdlbcpp_amd as td
+from oracle import oracle as O
+from tests.helpers import ReferenceLoop, assert_close_moments
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def case(msg):
+    print(f"[gpu case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("streaming", [td.STREAM_SIMPLE, td.STREAM_ESOTWIST], ids=["Simple", "Esotwist"])
+@pytest.mark.parametrize("layout", [td.LAYOUT_IJKL, td.LAYOUT_LIJK], ids=["IJKL", "LIJK"])
+def test_checkpoint_write_read_round_trip(tmp_path, streaming, layout):
+    case(f"checkpoint round trip {streaming} {layout}")
+    T = 36
+    flow = td.flow_for_tank(T)
+    a = td.ComputeUnit(T, layout=layout, streaming=streaming, **flow)
+    a.initialise(1.0)
+    a.bind_rushton(td.rushton_defaults(T))
+    a.run(7)  # lazy state: the write must materialise the reference's moments
+    rc = td.lib().tdlb_cu_checkpoint_write(a._h, str(tmp_path).encode(), b"device", b'{"x": 1}')
+    assert rc == 0, td.lib().tdlb_last_error()
+    Q, F, Nu = a.get(td.FIELD_Q), a.get(td.FIELD_F), a.get(td.FIELD_NU)
+    # raw files hold the arrays exactly (names DiskOutputTree.h:560-590)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "checkpoint_grid.0.0.0.device.Q", dtype=np.float32), Q)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "checkpoint_grid.0.0.0.device.F", dtype=np.float32), F)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "checkpoint_grid.0.0.0.device.Nu", dtype=np.float32), Nu)
+    assert json.loads((tmp_path / "AllParams.0.0.0.device.json").read_text()) == {"x": 1}
+    b = td.ComputeUnit(T, layout=layout, streaming=streaming, **flow)
+    missing = td.ctypes.c_int(-1)
+    assert td.lib().tdlb_cu_checkpoint_read(b._h, str(tmp_path).encode(), b"device", td.ctypes.byref(missing)) == 0
+    assert missing.value == 0
+    b.even_step = a.even_step
+    for f in (td.FIELD_Q, td.FIELD_F, td.FIELD_NU):
+        np.testing.assert_array_equal(b.get(f), a.get(f))
+    # both continue identically from the restored state
+    for cu in (a, b):
+        cu.collision()
+        cu.streaming_push()
+        cu.moments()
+    np.testing.assert_array_equal(b.get(td.FIELD_Q), a.get(td.FIELD_Q))
+
+
+def test_checkpoint_read_reports_missing_and_wrong_size(tmp_path):
+    case("checkpoint missing / wrong size")
+    a = td.ComputeUnit(8, layout=td.LAYOUT_LIJK, streaming=td.STREAM_SIMPLE)
+    a.initialise(1.0)
+    missing = td.ctypes.c_int(0)
+    assert td.lib().tdlb_cu_checkpoint_read(a._h, str(tmp_path).encode(), b"device", td.ctypes.byref(missing)) == 0
+    assert missing.value == 7
+    (tmp_path / "checkpoint_grid.0.0.0.device.Q").write_bytes(b"\0" * 100)
+    assert td.lib().tdlb_cu_checkpoint_read(a._h, str(tmp_path).encode(), b"device", None) == -5
+
+
+def _config1_input(tmp_path):
+    with open(os.path.join(ROOT, "tests", "golden", "input_debug_gridx60_numSteps20.json")) as fh:
+        j = json.load(fh)
+    j["FlowParams"].update(initialRho=1.0, cs0=0.1, useLES=True, streaming="Simple", uav=0.1,
+                           reMNonDimensional=7300.0, alpha=0.97, beta=1.9, g3=0.8)
+    j["RunningParams"].update(num_steps=20, step=1, angle=0.0, doubleResolutionAtStep=1000)
+    j["CheckpointParams"].update(checkpointRepeat=10, checkpointWriteRootDir=str(tmp_path),
+                                 checkpointWriteDirAppendTime=False, checkpointWriteDirPrefix="cfg1")
+    p = tmp_path / "input.json"
+    p.write_text(json.dumps(j))
+    return p, j
+
+
+@pytest.mark.parametrize("mode", ["fused", "per-call"])
+def test_driver_config1_matches_oracle(tmp_path, mode):
+    """tdlb_run on configuration 1: the checkpoint written at running.step 20
+    (after 19 main-loop iterations) equals the oracle's Q, F and Nu to the bar."""
+    case(f"driver config 1 {mode}")
+    inp, j = _config1_input(tmp_path)
+    args = [os.path.join(ROOT, "tdlbcpp_amd", "tdlb_run"), "-i", str(inp)]
+    if mode == "per-call":
+        args.append("--per-call")
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    d = tmp_path / "cfg1_checkpoint" / "cfg1_checkpoint_step_20"
+    allp = json.loads((d / "AllParams.0.0.0.device.json").read_text())
+    assert allp["RunningParams"]["step"] == 20
+    assert allp["ComputeUnitParams"]["x"] == 60 and allp["BinFileParams"]["structName"] == "checkpoint"
+    nu = allp["FlowParams"]["nu"]
+    T = 60
+    o = O.OracleCU(T, dtype=O.F32, layout=O.IJKL, streaming=O.SIMPLE, use_les=True, initialRho=1.0, cs0=0.1,
+                   g3=0.8, nu=nu, alpha=0.97, beta=1.9)
+    o.initialise(1.0)
+    ref = ReferenceLoop(o, td.rushton_defaults(T, tip_speed=0.1, startup_steps=30))
+    ref.run(19)
+    q = np.fromfile(d / "checkpoint_grid.0.0.0.device.Q", dtype=np.float32)
+    f = np.fromfile(d / "checkpoint_grid.0.0.0.device.F", dtype=np.float32)
+    n = np.fromfile(d / "checkpoint_grid.0.0.0.device.Nu", dtype=np.float32)
+    assert_close_moments(q, o.get(O.FIELD_Q), what="Q")
+    assert_close_moments(f, o.get(O.FIELD_F), what="F", rho0=1e-3)
+    assert_close_moments(n, o.get(O.FIELD_NU), what="Nu", rho0=1e-3)
+    print(f"[gpu case] config 1 bit-identical Q: {np.array_equal(q, o.get(O.FIELD_Q))}", flush=True)
+    assert len(glob.glob(str(tmp_path / "cfg1_checkpoint" / "*"))) == 2
