@@ -59,13 +59,18 @@ class Hip:
         assert self.L.hipDeviceSynchronize() == 0
 
 
-def bytes_per_cell(prec_bytes=4, les=True):
-    return 2 * 18 * prec_bytes + (2 * prec_bytes if les else 0)
+def bytes_per_cell(pop_bytes=4, les=True, nu_bytes=4):
+    """Algorithmic bytes of one fused stream-collide cell update: 18 populations
+    read and written, plus Nu read and written with LES."""
+    return 2 * 18 * pop_bytes + (2 * nu_bytes if les else 0)
 
 
-def run_variant(td, hip, grid, streaming, layout, steps, warmup, re=1.0e5, uav=0.1):
+POP_BYTES = {0: 4, 1: 8, 2: 2, 3: 2}
+
+
+def run_variant(td, hip, grid, streaming, layout, steps, warmup, precision=0, re=1.0e5, uav=0.1):
     flow = td.flow_for_tank(grid, re=re, uav=uav, cs0=0.1, use_les=1)
-    cu = td.ComputeUnit(grid, layout=layout, streaming=streaming, precision=td.PREC_F32, **flow)
+    cu = td.ComputeUnit(grid, layout=layout, streaming=streaming, precision=precision, **flow)
     rp = td.rushton_defaults(grid, tip_speed=uav, startup_steps=30)
     t0 = time.time()
     cu.initialise(1.0)
@@ -89,7 +94,8 @@ def run_variant(td, hip, grid, streaming, layout, steps, warmup, re=1.0e5, uav=0
                collide_ms=coll_ms / max(ncoll, 1), forcing_ms=force_ms / max(nforce, 1),
                ib_points_static=n_static, ib_points_rotating=n_rot, forcing_levels=n_levels,
                setup_s=round(setup_s, 2))
-    res["collide_gbs"] = bytes_per_cell() * cells / (res["collide_ms"] / 1e3) / 1e9
+    res["bytes_per_cell"] = bytes_per_cell(POP_BYTES[precision], nu_bytes=8 if precision == 1 else 4)
+    res["collide_gbs"] = res["bytes_per_cell"] * cells / (res["collide_ms"] / 1e3) / 1e9
     cu.close()
     return res
 
@@ -121,34 +127,61 @@ def cpu_baseline(sample_grid=96, max_seconds=20.0):
                        f"{steps} main-loop steps in {el:.1f} s on one host core")
 
 
+def copy_calibration(gib):
+    """A plain device copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel
+    give the factor between those counters and bytes moved."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    el = time.time() - t0
+    print(json.dumps({"copy_bytes_each_way": n * 4, "copies": 5, "gbs": round(2 * 5 * n * 4 / el / 1e9, 1)}),
+          flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--grid", type=int, default=512)
-    ap.add_argument("--variants", default="all", help="all or headline")
+    ap.add_argument("--variants", default="all", help="all, precision (fp32 + fp16/bf16 Esotwist/LIJK) or headline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only copy GIB GiB device-to-device 5 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs one ComputeUnit on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU slab decomposition is designed, not built: see DESIGN.md)", file=sys.stderr)
         return 2
+    if args.copy_calibration > 0:
+        return copy_calibration(args.copy_calibration)
 
     import tdlbcpp_amd as td
     hip = Hip()
-    variants = [("Esotwist", "LIJK", td.STREAM_ESOTWIST, td.LAYOUT_LIJK)]
+    F32, F16, BF16 = td.PREC_F32, td.PREC_F16, td.PREC_BF16
+    variants = [("Esotwist/LIJK", td.STREAM_ESOTWIST, td.LAYOUT_LIJK, F32)]
+    if args.variants in ("all", "precision"):
+        variants += [("Esotwist/LIJK/fp16", td.STREAM_ESOTWIST, td.LAYOUT_LIJK, F16),
+                     ("Esotwist/LIJK/bf16", td.STREAM_ESOTWIST, td.LAYOUT_LIJK, BF16)]
     if args.variants == "all":
-        variants += [("Esotwist", "IJKL", td.STREAM_ESOTWIST, td.LAYOUT_IJKL),
-                     ("Simple", "LIJK", td.STREAM_SIMPLE, td.LAYOUT_LIJK),
-                     ("Simple", "IJKL", td.STREAM_SIMPLE, td.LAYOUT_IJKL)]
+        variants += [("Esotwist/IJKL", td.STREAM_ESOTWIST, td.LAYOUT_IJKL, F32),
+                     ("Simple/LIJK", td.STREAM_SIMPLE, td.LAYOUT_LIJK, F32),
+                     ("Simple/IJKL", td.STREAM_SIMPLE, td.LAYOUT_IJKL, F32),
+                     ("Simple/LIJK/fp16", td.STREAM_SIMPLE, td.LAYOUT_LIJK, F16)]
     results = {}
-    for name, lay, st, ly in variants:
-        r = run_variant(td, hip, args.grid, st, ly, args.steps, args.warmup)
-        results[f"{name}/{lay}"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
-        print(f"[bench] {name}/{lay}: {r['mlups']:.0f} MLUPS, {r['ms_per_step']:.3f} ms/step, "
+    for name, st, ly, prec in variants:
+        r = run_variant(td, hip, args.grid, st, ly, args.steps, args.warmup, precision=prec)
+        results[name] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+        print(f"[bench] {name}: {r['mlups']:.0f} MLUPS, {r['ms_per_step']:.3f} ms/step, "
               f"collide {r['collide_ms']:.3f} ms ({r['collide_gbs']:.0f} GB/s), forcing {r['forcing_ms']:.3f} ms",
               file=sys.stderr, flush=True)
     head = results["Esotwist/LIJK"]
@@ -166,6 +199,8 @@ def main():
         "config": {"workload": f"rushton_tank_{args.grid}cubed_esotwist_lijk_fp32_les", "grid": args.grid,
                    "streaming": "Esotwist", "layout": "LIJK", "precision": "fp32", "les_cs0": 0.1,
                    "reynolds": 1.0e5, "tip_speed": 0.1, "variants": results},
+        "fp16_store_speedup": (round(results["Esotwist/LIJK/fp16"]["mlups"] / head["mlups"], 3)
+                               if "Esotwist/LIJK/fp16" in results else None),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_esotwist<float,LIJK,FUSED>",

@@ -108,6 +108,8 @@ def lib():
             "tdlb_cu_forcing_info": ([vp, P(i64), P(i64), P(i64)], i32),
             "tdlb_cu_profile": ([vp, i32], i32),
             "tdlb_cu_profile_read": ([vp, P(d), P(i64), P(d), P(i64)], i32),
+            "tdlb_cu_reductions": ([vp, P(d)], i32),
+            "tdlb_cu_macroscopic": ([vp, vp, ctypes.c_size_t], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -345,3 +347,17 @@ class ComputeUnit:
         a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         check(lib().tdlb_cu_forcing_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def reductions(self):
+        """FP64 device sums: {"mass", "momentum" (3), "force" (3), "torque"}."""
+        out = (ctypes.c_double * 8)()
+        check(lib().tdlb_cu_reductions(self._h, out))
+        v = list(out)
+        return {"mass": v[0], "momentum": tuple(v[1:4]), "force": tuple(v[4:7]), "torque": v[7]}
+
+    def macroscopic(self):
+        """(x, y, z, 4) array of {rho, ux, uy, uz} over the interior cells."""
+        nx, ny, nz = self.xg - 2, self.yg - 2, self.zg - 2
+        out = np.empty((nx, ny, nz, 4), dtype=self.np_dtype)
+        check(lib().tdlb_cu_macroscopic(self._h, out.ctypes.data, out.nbytes))
+        return out

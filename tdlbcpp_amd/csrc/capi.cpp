@@ -140,6 +140,7 @@ struct tdlb_cu {
     int cur = 0;
     void* M = nullptr;   // explicit Q of the reduced-precision storage (float)
     float rho0 = 1.0f;   // rest density of the deviation storage
+    double* red = nullptr;  // FP64 partial sums + result of tdlb_cu_reductions
     void* F = nullptr;
     void* Nu = nullptr;
     uint8_t* O = nullptr;
@@ -206,7 +207,7 @@ int set_device(tdlb_cu* cu) {
 
 void release(tdlb_cu* cu) {
     void* ptrs[] = {cu->Q[0], cu->Q[1], cu->M, cu->F, cu->Nu, cu->O, cu->fmask[0], cu->fmask[1],
-                    cu->gmask, cu->static_bits, cu->static_base};
+                    cu->gmask, cu->static_bits, cu->static_base, cu->red};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     cu->pts.release();
@@ -255,6 +256,16 @@ int materialise(tdlb_cu* cu) {
     }
     cu->phase = PH_EXPLICIT;
     return TDLB_OK;
+}
+
+// Where density and momentum of a cell come from in the current phase.
+struct CellSource {
+    int prec, momsrc, even;
+    const void* Q;
+};
+CellSource cell_source(const tdlb_cu* cu) {
+    if (cu->phase == PH_EXPLICIT) return {cu->eprec(), MOMSRC_STORED, cu->even ? 1 : 0, cu->eq()};
+    return {cu->prec(), cu->simple() ? MOMSRC_PULL_POPS : MOMSRC_ESO_POPS, cu->even ? 1 : 0, cu->Q[cu->cur]};
 }
 
 // the per-call operations work on the reference's Q directly
@@ -1150,6 +1161,46 @@ int tdlb_cu_solid_mask(tdlb_cu_t cu, uint8_t* out, size_t bytes) {
     if (e == hipSuccess) e = hipStreamSynchronize(cu->stream);
     (void)hipFree(tmp);
     if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("solid mask: ") + hipGetErrorString(e));
+    return TDLB_OK;
+}
+
+int tdlb_cu_reductions(tdlb_cu_t cu, double* out) {
+    if (!cu || !out) return fail(TDLB_ERR_INVALID, "null argument");
+    CHK(set_device(cu));
+    if (!cu->red && hipMalloc((void**)&cu->red, sizeof(double) * (kReduceBlocks + 1) * kReduceVals) != hipSuccess)
+        return fail(TDLB_ERR_NOMEM, "reduction buffer");
+    ReduceArgs ra{};
+    ra.ghost = (int)cu->prm.ghost;
+    ra.r2 = -1.0;
+    if (cu->bound) {
+        const double r = cu->rp.blade_outer_radius + 2.0;
+        ra.cx = cu->rp.center_x;
+        ra.cz = cu->rp.center_z;
+        ra.r2 = r * r;
+    }
+    const CellSource cs = cell_source(cu);
+    double* res = cu->red + (size_t)kReduceBlocks * kReduceVals;
+    HIPCHK(launch_reduce(cs.prec, cu->prm.layout, cu->prm.streaming, cs.momsrc, cs.even, cs.Q, cu->F,
+                         cu->fmask[cu->fm], cu->g, ra, cu->rho0, cu->red, res, cu->stream));
+    HIPCHK(hipMemcpyAsync(out, res, sizeof(double) * kReduceVals, hipMemcpyDeviceToHost, cu->stream));
+    HIPCHK(hipStreamSynchronize(cu->stream));
+    return TDLB_OK;
+}
+
+int tdlb_cu_macroscopic(tdlb_cu_t cu, void* out, size_t bytes) {
+    if (!cu || !out) return fail(TDLB_ERR_INVALID, "null argument");
+    const size_t need = (size_t)cu->g.xg1 * cu->g.yg1 * cu->g.zg1 * 4 * cu->tsize;
+    if (bytes != need) return fail(TDLB_ERR_INVALID, "macroscopic: buffer size must be " + std::to_string(need));
+    CHK(set_device(cu));
+    void* tmp = nullptr;
+    if (hipMalloc(&tmp, need) != hipSuccess) return fail(TDLB_ERR_NOMEM, "macroscopic staging");
+    const CellSource cs = cell_source(cu);
+    hipError_t e = launch_macroscopic(cs.prec, cu->prm.layout, cu->prm.streaming, cs.momsrc, cs.even, cs.Q, cu->F,
+                                      cu->fmask[cu->fm], cu->g, cu->rho0, tmp, cu->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, need, hipMemcpyDeviceToHost, cu->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(cu->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("macroscopic: ") + hipGetErrorString(e));
     return TDLB_OK;
 }
 

@@ -105,6 +105,24 @@ hipError_t launch_ib_moments_points(int prec, int layout, int streaming, int mom
 hipError_t launch_force_level_cached(int prec, const PointsDev& p, int64_t begin, int64_t count,
                                      const int32_t* sidx, int64_t cache_base, const void* cache, void* F,
                                      const Grid& g, double alfa, double beta, hipStream_t s);
+// FP64 sums over the interior cells: {rho, jx, jy, jz, Fx, Fy, Fz, torque}.  The
+// torque sum x*Fz - z*Fx runs over cells whose centre (x, z) relative to the
+// axis (cx, cz) lies within r2 (squared radius; negative: none).
+constexpr int kReduceVals = 8;
+constexpr int kReduceBlocks = 1024;
+struct ReduceArgs {
+    double cx, cz, r2;
+    int ghost;
+};
+// partial: kReduceBlocks * kReduceVals doubles; out: kReduceVals doubles (device)
+hipError_t launch_reduce(int prec, int layout, int streaming, int momsrc, int even, const void* Q, const void* F,
+                         const uint64_t* fmask, const Grid& g, const ReduceArgs& ra, float rho0, double* partial,
+                         double* out, hipStream_t s);
+// {rho, ux, uy, uz} per interior cell in T, interior cells in (i, j, k) order
+hipError_t launch_macroscopic(int prec, int layout, int streaming, int momsrc, int even, const void* Q,
+                              const void* F, const uint64_t* fmask, const Grid& g, float rho0, void* out,
+                              hipStream_t s);
+
 hipError_t launch_gen_blades(int prec, const BladeLayout& bl, const BladeStep& st, const PointsDev& out,
                              int64_t offset, hipStream_t s);
 hipError_t launch_solid_mask(double cx, double cz, double radius, uint8_t* out, const Grid& g, int ghost,

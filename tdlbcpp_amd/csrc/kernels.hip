@@ -13,6 +13,8 @@
 //   forcing             Forcing.hpp:56-176
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device.h"
 #include "es_collide.h"
 #include "kernels.h"
@@ -106,16 +108,13 @@ __global__ void k_copy_bytes(uint8_t* __restrict__ d, const uint8_t* __restrict_
 // mout receives the moments of MODE_MOM: Q itself (in place) when populations
 // are stored as T, a separate T array in the reference layout otherwise.
 template <typename PR, int L, int MODE, int EVEN>
-__global__ __launch_bounds__(kBlock) void k_esotwist(typename PR::S* __restrict__ Q, typename PR::T* mout,
-                                                     const typename PR::T* __restrict__ F,
-                                                     typename PR::T* __restrict__ Nu,
-                                                     const uint64_t* __restrict__ fmask, Grid g,
-                                                     CollideParams<typename PR::T> P, uint32_t nkb, float rho0) {
+__device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, typename PR::T* mout,
+                                              const typename PR::T* __restrict__ F, typename PR::T* __restrict__ Nu,
+                                              const uint64_t* __restrict__ fmask, const Grid& g,
+                                              const CollideParams<typename PR::T>& P, float rho0, int i, int j,
+                                              int k) {
     using T = typename PR::T;
     using D = typename PR::D;
-    const RowBlock rb = row_block(g, blockIdx.x, nkb, blockDim.x);
-    const int i = rb.i, j = rb.j, k = rb.k0 + (int)threadIdx.x;
-    if (k > g.zg1) return;
     const uint32_t n = cell_index(g, i, j, k);
     const int32_t di = (i == g.xg1) ? -(int32_t)((uint32_t)(g.xg1 - 1) * g.yz) : (int32_t)g.yz;
     const int32_t dj = (j == g.yg1) ? -(g.yg1 - 1) * g.zg : g.zg;
@@ -173,23 +172,32 @@ __global__ __launch_bounds__(kBlock) void k_esotwist(typename PR::S* __restrict_
     }
 }
 
+template <typename PR, int L, int MODE, int EVEN>
+__global__ __launch_bounds__(kBlock) void k_esotwist(typename PR::S* __restrict__ Q, typename PR::T* mout,
+                                                     const typename PR::T* __restrict__ F,
+                                                     typename PR::T* __restrict__ Nu,
+                                                     const uint64_t* __restrict__ fmask, Grid g,
+                                                     CollideParams<typename PR::T> P, uint32_t nkb, float rho0) {
+    const RowBlock rb = row_block(g, blockIdx.x, nkb, blockDim.x);
+    const int k = rb.k0 + (int)threadIdx.x;
+    if (k > g.zg1) return;
+    esotwist_cell<PR, L, MODE, EVEN>(Q, mout, F, Nu, fmask, g, P, rho0, rb.i, rb.j, k);
+}
+
 // ------------------------------------------------------------------ Simple
 // In-place collide / moments, or pull from src (the two-lattice form of the
 // reference's in-place push: slot d of an interior cell c takes src[c-e_d][d]
 // when c-e_d is interior and keeps src[c][d] otherwise).
 template <typename PR, int L, int MODE>
-__global__ __launch_bounds__(kBlock) void k_simple(const typename PR::S* src,
-                                                   typename PR::S* dst,  // equal to src for in-place modes
-                                                   typename PR::T* mout,  // moments output of the MOM modes
-                                                   const typename PR::T* __restrict__ F,
-                                                   typename PR::T* __restrict__ Nu,
-                                                   const uint64_t* __restrict__ fmask, Grid g,
-                                                   CollideParams<typename PR::T> P, uint32_t nkb, float rho0) {
+__device__ __forceinline__ void simple_cell(const typename PR::S* src,
+                                            typename PR::S* dst,   // equal to src for in-place modes
+                                            typename PR::T* mout,  // moments output of the MOM modes
+                                            const typename PR::T* __restrict__ F, typename PR::T* __restrict__ Nu,
+                                            const uint64_t* __restrict__ fmask, const Grid& g,
+                                            const CollideParams<typename PR::T>& P, float rho0, int i, int j,
+                                            int k) {
     using T = typename PR::T;
     using D = typename PR::D;
-    const RowBlock rb = row_block(g, blockIdx.x, nkb, blockDim.x);
-    const int i = rb.i, j = rb.j, k = rb.k0 + (int)threadIdx.x;
-    if (k > g.zg1) return;
     const uint32_t n = cell_index(g, i, j, k);
 
     T v[kQ];
@@ -239,6 +247,18 @@ __global__ __launch_bounds__(kBlock) void k_simple(const typename PR::S* src,
     if (P.use_les) Nu[n] = nuc;
 #pragma unroll
     for (int d = 0; d < kQ; d++) store_pop<PR>(dst, qaddr<L>(g, n, d), d, p[d], rho0);
+}
+
+template <typename PR, int L, int MODE>
+__global__ __launch_bounds__(kBlock) void k_simple(const typename PR::S* src, typename PR::S* dst,
+                                                   typename PR::T* mout, const typename PR::T* __restrict__ F,
+                                                   typename PR::T* __restrict__ Nu,
+                                                   const uint64_t* __restrict__ fmask, Grid g,
+                                                   CollideParams<typename PR::T> P, uint32_t nkb, float rho0) {
+    const RowBlock rb = row_block(g, blockIdx.x, nkb, blockDim.x);
+    const int k = rb.k0 + (int)threadIdx.x;
+    if (k > g.zg1) return;
+    simple_cell<PR, L, MODE>(src, dst, mout, F, Nu, fmask, g, P, rho0, rb.i, rb.j, k);
 }
 
 // ------------------------------------------------------------------ data movement on every cell
@@ -722,6 +742,104 @@ __global__ __launch_bounds__(kBlock) void k_force_level_cached(PointsDev p, int6
     }
 }
 
+// ------------------------------------------------------------------ FP64 sums and macroscopic fields
+// Both read density and momentum through cell_rho_j, so they work on stored
+// moments and on the populations of the fused path alike, without changing
+// the state.  Interior cells are enumerated t = ((i-1)*y + (j-1))*z + (k-1).
+__device__ __forceinline__ void interior_cell(const Grid& g, uint64_t t, int& i, int& j, int& k) {
+    k = (int)(t % (uint64_t)g.zg1) + 1;
+    const uint64_t r = t / (uint64_t)g.zg1;
+    j = (int)(r % (uint64_t)g.yg1) + 1;
+    i = (int)(r / (uint64_t)g.yg1) + 1;
+}
+
+template <typename T>
+__device__ __forceinline__ void cell_force(const T* __restrict__ F, const uint64_t* __restrict__ fmask,
+                                           const Grid& g, int i, int j, int k, T& fx, T& fy, T& fz) {
+    fx = fy = fz = T(0);
+    if (mask_test(fmask, g, i, j, k)) {
+        const uint64_t c = cell_index(g, i, j, k);
+        fx = F[3 * c + 0];
+        fy = F[3 * c + 1];
+        fz = F[3 * c + 2];
+    }
+}
+
+// Per block: {sum rho, sum jx, jy, jz, sum Fx, Fy, Fz, torque} in FP64.  The
+// grid is fixed, so the partial sums and their combination are deterministic.
+template <typename PR, int L, int STREAM, int MOMSRC, int EVEN>
+__global__ __launch_bounds__(kBlock) void k_reduce_cells(const typename PR::S* __restrict__ Q,
+                                                         const typename PR::T* __restrict__ F,
+                                                         const uint64_t* __restrict__ fmask, Grid g, ReduceArgs ra,
+                                                         float rho0, double* __restrict__ partial) {
+    using T = typename PR::T;
+    double acc[kReduceVals];
+#pragma unroll
+    for (int v = 0; v < kReduceVals; v++) acc[v] = 0.0;
+    const uint64_t n = (uint64_t)g.xg1 * g.yg1 * g.zg1;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        interior_cell(g, t, i, j, k);
+        T rho, jx, jy, jz, fx, fy, fz;
+        cell_rho_j<PR, L, STREAM, MOMSRC, EVEN>(Q, g, i, j, k, rho, jx, jy, jz, rho0);
+        cell_force(F, fmask, g, i, j, k, fx, fy, fz);
+        acc[0] += (double)rho;
+        acc[1] += (double)jx;
+        acc[2] += (double)jy;
+        acc[3] += (double)jz;
+        acc[4] += (double)fx;
+        acc[5] += (double)fy;
+        acc[6] += (double)fz;
+        const double x = (double)(i - ra.ghost) - ra.cx, z = (double)(k - ra.ghost) - ra.cz;
+        if (x * x + z * z <= ra.r2) acc[7] += x * (double)fz - z * (double)fx;
+    }
+    __shared__ double red[kReduceVals][kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int v = 0; v < kReduceVals; v++) {
+        double a = acc[v];
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+        if (lane == 0) red[v][wave] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < kReduceVals) {
+        double a = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) a += red[threadIdx.x][w];
+        partial[(uint64_t)blockIdx.x * kReduceVals + threadIdx.x] = a;
+    }
+}
+
+__global__ void k_reduce_final(const double* __restrict__ partial, int nblocks, double* __restrict__ out) {
+    if (threadIdx.x >= kReduceVals) return;
+    double a = 0.0;
+    for (int b = 0; b < nblocks; b++) a += partial[(uint64_t)b * kReduceVals + threadIdx.x];
+    out[threadIdx.x] = a;
+}
+
+// {rho, ux, uy, uz} per interior cell, u as QVec::velocity(F) forms it
+// (QVec.hpp:125-131: (1.0 / rho) * (j + 0.5 * F), double literals).
+template <typename PR, int L, int STREAM, int MOMSRC, int EVEN>
+__global__ __launch_bounds__(kBlock) void k_macroscopic(const typename PR::S* __restrict__ Q,
+                                                        const typename PR::T* __restrict__ F,
+                                                        const uint64_t* __restrict__ fmask, Grid g, float rho0,
+                                                        typename PR::T* __restrict__ out) {
+    using T = typename PR::T;
+    using D = typename PR::D;
+    const uint64_t n = (uint64_t)g.xg1 * g.yg1 * g.zg1;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        interior_cell(g, t, i, j, k);
+        T rho, jx, jy, jz, fx, fy, fz;
+        cell_rho_j<PR, L, STREAM, MOMSRC, EVEN>(Q, g, i, j, k, rho, jx, jy, jz, rho0);
+        cell_force(F, fmask, g, i, j, k, fx, fy, fz);
+        const D inv = D(1.0) / (D)rho;
+        out[4 * t + 0] = rho;
+        out[4 * t + 1] = (T)(inv * ((D)jx + D(0.5) * (D)fx));
+        out[4 * t + 2] = (T)(inv * ((D)jy + D(0.5) * (D)fy));
+        out[4 * t + 3] = (T)(inv * ((D)jz + D(0.5) * (D)fz));
+    }
+}
+
 // ------------------------------------------------------------------ geometry on the device
 template <typename T>
 __global__ void k_gen_blades(BladeLayout bl, BladeStep st, PointsDev out, int64_t offset, int64_t count) {
@@ -1099,6 +1217,75 @@ hipError_t launch_force_level_cached(int prec, const PointsDev& p, int64_t begin
                                cache_base, c, (T*)F, g, (T)alfa, (T)beta);
     });
     return hipGetLastError();
+}
+
+namespace {
+// fn(L, STREAM, MOMSRC, EVEN) with compile-time values (std::integral_constant)
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+template <typename PR, int L, int STREAM, typename Fn>
+hipError_t with_source(int momsrc, int even, Fn&& fn) {
+    if (momsrc == MOMSRC_STORED) {
+        if constexpr (PR::kShift) return hipErrorInvalidValue;
+        else return even ? fn(IC<L>{}, IC<STREAM>{}, IC<MOMSRC_STORED>{}, IC<1>{})
+                         : fn(IC<L>{}, IC<STREAM>{}, IC<MOMSRC_STORED>{}, IC<0>{});
+    }
+    if (momsrc == MOMSRC_ESO_POPS) {
+        if constexpr (STREAM != StreamEsotwist) return hipErrorInvalidValue;
+        else return even ? fn(IC<L>{}, IC<STREAM>{}, IC<MOMSRC_ESO_POPS>{}, IC<1>{})
+                         : fn(IC<L>{}, IC<STREAM>{}, IC<MOMSRC_ESO_POPS>{}, IC<0>{});
+    }
+    if constexpr (STREAM != StreamSimple) return hipErrorInvalidValue;
+    else return fn(IC<L>{}, IC<STREAM>{}, IC<MOMSRC_PULL_POPS>{}, IC<1>{});
+}
+
+template <typename PR, typename Fn>
+hipError_t with_cell_source(int layout, int streaming, int momsrc, int even, Fn&& fn) {
+    if (layout == LayoutIJKL)
+        return streaming == StreamEsotwist ? with_source<PR, LayoutIJKL, StreamEsotwist>(momsrc, even, fn)
+                                           : with_source<PR, LayoutIJKL, StreamSimple>(momsrc, even, fn);
+    return streaming == StreamEsotwist ? with_source<PR, LayoutLIJK, StreamEsotwist>(momsrc, even, fn)
+                                       : with_source<PR, LayoutLIJK, StreamSimple>(momsrc, even, fn);
+}
+}  // namespace
+
+hipError_t launch_reduce(int prec, int layout, int streaming, int momsrc, int even, const void* Q, const void* F,
+                         const uint64_t* fmask, const Grid& g, const ReduceArgs& ra, float rho0, double* partial,
+                         double* out, hipStream_t s) {
+    hipError_t e = hipErrorInvalidValue;
+    TDLB_WITH_PREC(prec, {
+        using S = typename PR::S;
+        using T = typename PR::T;
+        e = with_cell_source<PR>(layout, streaming, momsrc, even, [&](auto Lc, auto Sc, auto Mc, auto Ec) {
+            hipLaunchKernelGGL((k_reduce_cells<PR, decltype(Lc)::value, decltype(Sc)::value, decltype(Mc)::value,
+                                               decltype(Ec)::value>),
+                               dim3(kReduceBlocks), dim3(kBlock), 0, s, (const S*)Q, (const T*)F, fmask, g, ra, rho0,
+                               partial);
+            return hipGetLastError();
+        });
+    });
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, s, partial, kReduceBlocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_macroscopic(int prec, int layout, int streaming, int momsrc, int even, const void* Q,
+                              const void* F, const uint64_t* fmask, const Grid& g, float rho0, void* out,
+                              hipStream_t s) {
+    const uint32_t nb = grid_blocks((uint64_t)g.xg1 * g.yg1 * g.zg1);
+    hipError_t e = hipErrorInvalidValue;
+    TDLB_WITH_PREC(prec, {
+        using S = typename PR::S;
+        using T = typename PR::T;
+        e = with_cell_source<PR>(layout, streaming, momsrc, even, [&](auto Lc, auto Sc, auto Mc, auto Ec) {
+            hipLaunchKernelGGL((k_macroscopic<PR, decltype(Lc)::value, decltype(Sc)::value, decltype(Mc)::value,
+                                              decltype(Ec)::value>),
+                               dim3(nb), dim3(kBlock), 0, s, (const S*)Q, (const T*)F, fmask, g, rho0, (T*)out);
+            return hipGetLastError();
+        });
+    });
+    return e;
 }
 
 hipError_t launch_gen_blades(int prec, const BladeLayout& bl, const BladeStep& st, const PointsDev& out,

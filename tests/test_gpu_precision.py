@@ -1,0 +1,128 @@
+"""Macroscopic fields, FP64 device sums and the 16-bit population storage.
+
+rho/u follow QVec::velocity(F) (QVec.hpp:125-131) and must equal the oracle's
+moments pushed through the same formula bit for bit, in both handle phases.
+The FP64 sums are checked against numpy sums of the same data.  The 16-bit
+storage has no reference counterpart: it is checked against the fp64 build of
+the same kernels (velocity L2 over the fluid, power number, mass drift).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import tdlbcpp_amd as td
+from oracle import oracle as O
+from tdlbcpp_amd import precision as P
+from tests.helpers import ReferenceLoop, flow_for_tank
+
+pytestmark = pytest.mark.gpu
+
+LAYOUTS = [(td.LAYOUT_IJKL, O.IJKL), (td.LAYOUT_LIJK, O.LIJK)]
+STREAMS = [(td.STREAM_SIMPLE, O.SIMPLE), (td.STREAM_ESOTWIST, O.ESOTWIST)]
+
+
+def case(msg):
+    print(f"[gpu case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def host_macroscopic(o):
+    """{rho, ux, uy, uz} from the oracle's moments and F, QVec::velocity(F) in float."""
+    m = o.read_cells().astype(np.float32)
+    F = o.get(O.FIELD_F).reshape(o.xg, o.yg, o.zg, 3)[1:-1, 1:-1, 1:-1].astype(np.float32)
+    rho = m[..., 0].astype(np.float64)
+    out = np.empty(m.shape[:3] + (4,), dtype=np.float32)
+    out[..., 0] = m[..., 0]
+    for a in range(3):
+        out[..., 1 + a] = ((1.0 / rho) * (m[..., 1 + a].astype(np.float64) + 0.5 * F[..., a].astype(np.float64))
+                           ).astype(np.float32)
+    return out, m, F
+
+
+@pytest.mark.parametrize("stream", STREAMS, ids=["Simple", "Esotwist"])
+@pytest.mark.parametrize("layout", LAYOUTS, ids=["IJKL", "LIJK"])
+def test_macroscopic_and_sums_match_oracle(layout, stream):
+    T = 36
+    case(f"macroscopic T={T} {layout} {stream}")
+    oflow, gflow = flow_for_tank(T)
+    o = O.OracleCU(T, dtype=O.F32, layout=layout[1], streaming=stream[1], use_les=True, **oflow)
+    g = td.ComputeUnit(T, layout=layout[0], streaming=stream[0], precision=td.PREC_F32, **gflow)
+    rp = td.rushton_defaults(T)
+    o.initialise(1.0)
+    g.initialise(1.0)
+    ref = ReferenceLoop(o, rp)
+    g.bind_rushton(rp)
+    ref.run(9)
+    g.run(9)
+    want, m, F = host_macroscopic(o)
+    lazy = g.macroscopic()        # computed from the fused path's populations
+    red_lazy = g.reductions()
+    g.get(td.FIELD_Q)             # materialise
+    explicit = g.macroscopic()    # computed from the stored moments
+    red = g.reductions()
+    np.testing.assert_array_equal(lazy, want)
+    np.testing.assert_array_equal(explicit, want)
+    assert red == red_lazy
+    m64, F64 = m.astype(np.float64), F.astype(np.float64)
+    np.testing.assert_allclose(red["mass"], m64[..., 0].sum(), rtol=1e-13)
+    for a in range(3):
+        np.testing.assert_allclose(red["momentum"][a], m64[..., 1 + a].sum(), rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(red["force"][a], F64[..., a].sum(), rtol=1e-9, atol=1e-14)
+    i = np.arange(T)[:, None] - rp.center_x
+    k = np.arange(T)[None, :] - rp.center_z
+    inside = (i * i + k * k) <= (rp.blade_outer_radius + 2.0) ** 2
+    tq = (i[:, None, :] * F64[..., 2] - k[:, None, :] * F64[..., 0])[np.broadcast_to(inside[:, None, :], F64.shape[:3])]
+    np.testing.assert_allclose(red["torque"], tq.sum(), rtol=1e-9, atol=1e-14)
+    assert red["torque"] != 0.0
+
+
+@pytest.mark.parametrize("prec", [td.PREC_F16, td.PREC_BF16], ids=["fp16", "bf16"])
+def test_16bit_storage_rejects_per_call_operations(prec):
+    case(f"16-bit per-call rejection {prec}")
+    _, gflow = flow_for_tank(36)
+    g = td.ComputeUnit(36, layout=td.LAYOUT_LIJK, streaming=td.STREAM_ESOTWIST, precision=prec, **gflow)
+    g.initialise(1.0)
+    for call in (g.collision, g.moments, g.streaming_push, g.streaming_pull, g.bounce_back):
+        with pytest.raises(td.TdlbError):
+            call()
+    q = g.get(td.FIELD_Q)
+    assert q.dtype == np.float32 and q.size == g.size * 18
+
+
+@pytest.mark.parametrize("stream", STREAMS, ids=["Simple", "Esotwist"])
+@pytest.mark.parametrize("prec", [td.PREC_F16, td.PREC_BF16], ids=["fp16", "bf16"])
+def test_16bit_storage_tracks_fp64(prec, stream):
+    """A short tank run in 16-bit storage stays close to fp64 and conserves mass."""
+    T, steps = 48, 300
+    case(f"16-bit storage T={T} {steps} steps prec={prec} {stream}")
+    kw = dict(streaming=stream[0], startup_steps=50, samples=5)
+    ref = P.tank_run(T, td.PREC_F64, steps, **kw)
+    r = P.tank_run(T, prec, steps, **kw)
+    f32 = P.tank_run(T, td.PREC_F32, steps, **kw)
+    l2 = P.velocity_l2(r["mac"], ref["mac"], ref["fluid"])
+    l2_32 = P.velocity_l2(f32["mac"], ref["mac"], ref["fluid"])
+    drift = abs(r["mass"][-1] / r["mass0"] - 1.0)
+    print(f"[gpu case] velocity L2 vs fp64: {l2:.3e} (fp32 {l2_32:.3e}); Po {r['po']:.4f} vs {ref['po']:.4f}; "
+          f"mass drift {drift:.2e}", flush=True)
+    assert np.isfinite(r["mac"]).all()
+    bound = 2e-3 if prec == td.PREC_F16 else 3e-2
+    assert l2 <= bound
+    assert abs(r["po"] - ref["po"]) <= 0.05 * abs(ref["po"])
+    assert drift <= (1e-4 if prec == td.PREC_F16 else 1e-3)
+
+
+def test_fp64_rest_tank_conserves_mass_exactly():
+    """The fp64 build with the impeller at rest keeps the fluid at rest."""
+    T = 36
+    case("fp64 rest tank")
+    _, gflow = flow_for_tank(T, dtype=O.F64)
+    g = td.ComputeUnit(T, layout=td.LAYOUT_LIJK, streaming=td.STREAM_ESOTWIST, precision=td.PREC_F64, **gflow)
+    g.initialise(1.0)
+    rp = td.rushton_defaults(T, tip_speed=0.0)
+    g.bind_rushton(rp)
+    g.run(20)
+    mac = g.macroscopic()
+    assert np.abs(mac[..., 1:]).max() < 1e-14
+    red = g.reductions()
+    assert abs(red["mass"] - T ** 3) < 1e-9
