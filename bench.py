@@ -127,6 +127,70 @@ def cpu_baseline(sample_grid=96, max_seconds=20.0):
                        f"{steps} main-loop steps in {el:.1f} s on one host core")
 
 
+def batch_tanks(n_tanks=64, steps=2000, seed=1):
+    """Independent Rushton tanks: grid edge drawn from {48, 64, 96, 128, 160},
+    tip speed U[0.05, 0.12], Re U[2e3, 2e4] (numpy PCG64 seeded with `seed`)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    edges = rng.choice([48, 64, 96, 128, 160], size=n_tanks)
+    tips = rng.uniform(0.05, 0.12, size=n_tanks)
+    res = rng.uniform(2e3, 2e4, size=n_tanks)
+    return [dict(edge=int(e), tip=float(t), re=float(r), steps=steps) for e, t, r in zip(edges, tips, res)]
+
+
+def make_tank(td, spec, precision, streaming, layout):
+    flow = td.flow_for_tank(spec["edge"], re=spec["re"], uav=spec["tip"], cs0=0.1, use_les=1)
+    cu = td.ComputeUnit(spec["edge"], layout=layout, streaming=streaming, precision=precision, **flow)
+    cu.initialise(1.0)
+    cu.bind_rushton(td.rushton_defaults(spec["edge"], tip_speed=spec["tip"], startup_steps=100))
+    return cu
+
+
+def run_batch_workload(td, hip, n_tanks, steps, precision, streaming, layout, solo_sample):
+    """Batched (one graph-replayed launch sequence for all tanks) against
+    one-at-a-time runs of the same tanks."""
+    specs = batch_tanks(n_tanks, steps)
+    cells = sum(s["edge"] ** 3 for s in specs)
+    t0 = time.time()
+    tanks = [make_tank(td, s, precision, streaming, layout) for s in specs]
+    setup = time.time() - t0
+    b = td.Batch(precision, layout, streaming)
+    t0 = time.time()
+    for cu, s in zip(tanks, specs):
+        b.add(cu, s["steps"])
+    tune = time.time() - t0
+    b.run(2)  # settles each member and captures the graph
+    hip.device_sync()
+    t0 = time.time()
+    b.run(steps)
+    hip.device_sync()
+    el = time.time() - t0
+    info = b.info()
+    b.close()
+    done = steps - 2
+    batched = cells * done / el / 1e6
+    # one at a time: the same tanks, each alone for its steps (timed on a sample of the tanks)
+    sample = specs[:solo_sample] if solo_sample else specs
+    scells = sum(s["edge"] ** 3 for s in sample)
+    for cu in tanks:
+        cu.close()
+    solo_s = 0.0
+    for s in sample:
+        cu = make_tank(td, s, precision, streaming, layout)
+        cu.run(2)
+        hip.device_sync()
+        t0 = time.time()
+        cu.run(done)
+        hip.device_sync()
+        solo_s += time.time() - t0
+        cu.close()
+    solo = scells * done / solo_s / 1e6
+    return dict(tanks=n_tanks, steps=done, cells=cells, batched_mlups=batched, batched_s=el, solo_mlups=solo,
+                solo_sample_tanks=len(sample), speedup=batched / solo, setup_s=round(setup, 1),
+                autotune_s=round(tune, 2), launches_per_step=info["launches_per_step"],
+                tiles={int(e): t for e, t in zip([s["edge"] for s in specs], info["tiles"])})
+
+
 def copy_calibration(gib):
     """A plain device copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel
     give the factor between those counters and bytes moved."""
@@ -154,6 +218,12 @@ def main():
     ap.add_argument("--variants", default="all", help="all, precision (fp32 + fp16/bf16 Esotwist/LIJK) or headline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--workload", default="tank", choices=["tank", "batch"],
+                    help="tank: one 512^3 tank (headline); batch: independent tanks batched vs one at a time")
+    ap.add_argument("--tanks", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=2000)
+    ap.add_argument("--solo-sample", type=int, default=0,
+                    help="time one-at-a-time runs on the first N tanks only (0: all)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only copy GIB GiB device-to-device 5 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -167,6 +237,18 @@ def main():
 
     import tdlbcpp_amd as td
     hip = Hip()
+    if args.workload == "batch":
+        r = run_batch_workload(td, hip, args.tanks, args.batch_steps, td.PREC_F16, td.STREAM_ESOTWIST,
+                               td.LAYOUT_LIJK, args.solo_sample)
+        print(f"[bench] batch: {r['batched_mlups']:.0f} MLUPS batched vs {r['solo_mlups']:.0f} one at a time "
+              f"({r['speedup']:.2f}x)", file=sys.stderr, flush=True)
+        out = {"metric": METRIC, "value": round(r["batched_mlups"], 1), "unit": "MLUPS", "n_gpus": 1,
+               "steps": r["steps"], "warmup": 2, "ms_per_step": round(1e3 * r["batched_s"] / r["steps"], 4),
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f16-store/f32",
+               "data": "synthetic: Rushton tanks at rest, geometry from this library's generator",
+               "config": {"workload": f"batched_rushton_tanks_{args.tanks}_fp16_esotwist_lijk", **r}}
+        print(json.dumps(out), flush=True)
+        return 0
     F32, F16, BF16 = td.PREC_F32, td.PREC_F16, td.PREC_BF16
     variants = [("Esotwist/LIJK", td.STREAM_ESOTWIST, td.LAYOUT_LIJK, F32)]
     if args.variants in ("all", "precision"):

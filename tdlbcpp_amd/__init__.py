@@ -110,6 +110,14 @@ def lib():
             "tdlb_cu_profile_read": ([vp, P(d), P(i64), P(d), P(i64)], i32),
             "tdlb_cu_reductions": ([vp, P(d)], i32),
             "tdlb_cu_macroscopic": ([vp, vp, ctypes.c_size_t], i32),
+            "tdlb_batch_create": ([i32, i32, i32, i32, P(vp)], i32),
+            "tdlb_batch_destroy": ([vp], i32),
+            "tdlb_batch_set_options": ([vp, i32, i32], i32),
+            "tdlb_batch_add": ([vp, vp, u64, d, i64], i32),
+            "tdlb_batch_remove": ([vp, vp], i32),
+            "tdlb_batch_run": ([vp, i64, P(i64)], i32),
+            "tdlb_batch_member": ([vp, vp, P(u64), P(d), P(i64)], i32),
+            "tdlb_batch_info": ([vp, P(i64), P(i64), P(ctypes.c_int32), i64], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -361,3 +369,53 @@ class ComputeUnit:
         out = np.empty((nx, ny, nz, 4), dtype=self.np_dtype)
         check(lib().tdlb_cu_macroscopic(self._h, out.ctypes.data, out.nbytes))
         return out
+
+
+class Batch:
+    """Many bound ComputeUnits advanced together (tdlb_batch_*): one launch per
+    kernel of the main-loop step for all members, the step replayed from a HIP graph."""
+
+    def __init__(self, precision=PREC_F16, layout=LAYOUT_LIJK, streaming=STREAM_ESOTWIST, device_id=0,
+                 use_graph=True, autotune=True):
+        self._b = ctypes.c_void_p()
+        check(lib().tdlb_batch_create(precision, layout, streaming, device_id, ctypes.byref(self._b)))
+        check(lib().tdlb_batch_set_options(self._b, 1 if use_graph else 0, 1 if autotune else 0))
+        self.members = []
+
+    def add(self, cu, nsteps):
+        check(lib().tdlb_batch_add(self._b, cu._h, cu.step, cu.angle, nsteps))
+        self.members.append(cu)
+
+    def remove(self, cu):
+        check(lib().tdlb_batch_remove(self._b, cu._h))
+        self.members.remove(cu)
+
+    def member(self, cu):
+        """(step, angle, remaining) of a member."""
+        s, a, r = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_int64()
+        check(lib().tdlb_batch_member(self._b, cu._h, ctypes.byref(s), ctypes.byref(a), ctypes.byref(r)))
+        return s.value, a.value, r.value
+
+    def run(self, nsteps):
+        n = ctypes.c_int64()
+        check(lib().tdlb_batch_run(self._b, nsteps, ctypes.byref(n)))
+        for cu in self.members:
+            cu.step, cu.angle, _ = self.member(cu)
+        return n.value
+
+    def info(self):
+        n, l = ctypes.c_int64(), ctypes.c_int64()
+        tiles = (ctypes.c_int32 * max(1, len(self.members)))()
+        check(lib().tdlb_batch_info(self._b, ctypes.byref(n), ctypes.byref(l), tiles, len(self.members)))
+        return {"members": n.value, "launches_per_step": l.value, "tiles": list(tiles)[:len(self.members)]}
+
+    def close(self):
+        if getattr(self, "_b", None) and self._b.value:
+            check(lib().tdlb_batch_destroy(self._b))
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

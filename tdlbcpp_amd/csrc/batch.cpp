@@ -1,0 +1,542 @@
+// Batches of ComputeUnits on one GPU (no counterpart in the reference, which
+// runs one ComputeUnit per process).
+//
+// A batch holds bound handles of one precision family (float arithmetic),
+// streaming and layout.  tdlb_batch_run advances every member with one launch
+// per kernel of the main-loop step for the whole batch, the step captured once
+// in a HIP graph and replayed.  Each member runs exactly the arithmetic of
+// tdlb_cu_run on the same handle (same device functions, same order), so a
+// tank's result does not depend on the batch it ran in.
+//
+// Per-member tile shapes of the fused kernel come from an autotuner that times
+// each candidate on scratch lattices of the member's size (never on the
+// member's own state) and caches the choice per grid shape.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/tdlb.h"
+#include "handle.h"
+#include "kernels.h"
+#include "rushton.h"
+
+namespace {
+
+struct Member {
+    tdlb_cu* cu = nullptr;
+    uint64_t step = 0;   // RunningParams.step after the member's last completed step
+    double angle = 0.0;  // RunningParams.angle
+    int64_t remaining = 0;
+    int32_t tile = TILE_ROW64;
+    // filled per run
+    int64_t call = 0;     // steps left for this member in the current tdlb_batch_run
+    int64_t n_steps = 0;
+    std::vector<double> angles;  // angle after each step of the run
+};
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    int grow(size_t n) {  // returns 1 when reallocated
+        if (n <= cap) return 0;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t c = std::max<size_t>(n, 64) * 3 / 2;
+        if (hipMalloc((void**)&p, c * sizeof(T)) != hipSuccess) return -1;
+        cap = c;
+        return 1;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+using TileKey = std::tuple<int64_t, int64_t, int64_t, int, int, int>;
+std::mutex g_tile_mu;
+std::map<TileKey, int32_t> g_tiles;  // autotuned tile per grid shape, precision, layout, streaming
+
+}  // namespace
+
+struct tdlb_batch {
+    int prec = 0, layout = 0, streaming = 0, device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<Member> members;
+    bool use_graph = true;
+    bool autotune = true;
+
+    DevBuf<MemberDev> d_mem;
+    DevBuf<int64_t> d_tab;
+    DevBuf<LevelSeg> d_segs;
+    DevBuf<BladeStep> d_steps;
+
+    // plan of the last run and its graph
+    BatchPlan plan{};
+    std::vector<int64_t> lv_static, lv_blade, lv_threads;
+    std::vector<std::pair<tdlb_cu*, int32_t>> key;  // members (with budget) and their tiles
+    bool buffers_moved = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int64_t launches_per_step = 0;
+    std::vector<int> active_idx;  // members in the plan (indices into members)
+};
+
+namespace {
+
+double to_prec_f(const tdlb_cu* cu, double v) { return cu->tsize == 4 ? (double)(float)v : v; }
+
+void drop_graph(tdlb_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+MemberDev describe(const tdlb_batch* b, const Member& m) {
+    const tdlb_cu* cu = m.cu;
+    MemberDev md{};
+    md.g = cu->g;
+    if (b->prec == TDLB_PREC_F32)
+        md.P = make_collide_params<float, double>((float)cu->flow.nu, (float)cu->flow.cs0, (float)cu->flow.g3,
+                                                  cu->flow.use_les);
+    else
+        md.P = make_collide_params<float, float>((float)cu->flow.nu, (float)cu->flow.cs0, (float)cu->flow.g3,
+                                                 cu->flow.use_les);
+    md.rho0 = cu->rho0;
+    md.alfa = (float)cu->alfa;
+    md.beta = (float)cu->beta;
+    md.tile = m.tile;
+    md.Q[0] = cu->Q[0];
+    md.Q[1] = cu->Q[1];
+    md.F = static_cast<float*>(cu->F);
+    md.Nu = static_cast<float*>(cu->Nu);
+    md.fmask[0] = cu->fmask[0];
+    md.fmask[1] = cu->fmask[1];
+    md.static_base = cu->static_base;
+    md.pts = cu->pts.d;
+    md.n_static = cu->n_static;
+    md.n_rot = cu->n_rot;
+    md.scells = cu->static_set.cells;
+    md.sidx = cu->static_set.sidx;
+    md.n_scells = cu->static_set.n_cells;
+    md.scache = cu->static_set.cache;
+    md.rcache = cu->rot_cache;
+    md.blade = cu->blade;
+    md.even = cu->even ? 1 : 0;
+    md.cur = cu->cur;
+    md.fm = cu->fm;
+    return md;
+}
+
+// Tile of the fused kernel for one member: cached per shape, else timed on
+// scratch lattices of that shape (zero populations; the timing does not depend
+// on the values) when autotuning is on, else the row tile.
+int tune_tile(tdlb_batch* b, const tdlb_cu* cu, int32_t* out) {
+    const TileKey key{cu->g.xg1, cu->g.yg1, cu->g.zg1, b->prec, b->layout, b->streaming};
+    {
+        std::lock_guard<std::mutex> lk(g_tile_mu);
+        auto it = g_tiles.find(key);
+        if (it != g_tiles.end()) {
+            *out = it->second;
+            return TDLB_OK;
+        }
+    }
+    if (!b->autotune) {
+        *out = TILE_ROW64;
+        return TDLB_OK;
+    }
+    const size_t nq = cu->simple() ? 2 : 1;
+    void* q[2] = {nullptr, nullptr};
+    void *F = nullptr, *Nu = nullptr, *fm = nullptr, *dmem = nullptr, *dtab = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = TDLB_OK;
+    float best = 1e30f;
+    int32_t best_tile = TILE_ROW64;
+    auto cleanup = [&]() {
+        for (void* p : {q[0], q[1], F, Nu, fm, dmem, dtab})
+            if (p) (void)hipFree(p);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    bool ok = true;
+    for (size_t a = 0; a < nq && ok; a++) ok = hipMalloc(&q[a], cu->qs_bytes()) == hipSuccess;
+    ok = ok && hipMalloc(&F, cu->f_bytes()) == hipSuccess && hipMalloc(&Nu, cu->nu_bytes()) == hipSuccess &&
+         hipMalloc(&fm, cu->mask_bytes()) == hipSuccess && hipMalloc(&dmem, sizeof(MemberDev)) == hipSuccess &&
+         hipMalloc(&dtab, 2 * sizeof(int64_t)) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+         hipEventCreate(&e1) == hipSuccess;
+    if (!ok) {
+        cleanup();
+        return fail(TDLB_ERR_NOMEM, "autotune scratch lattices");
+    }
+    for (size_t a = 0; a < nq; a++) ok = ok && launch_zero(q[a], cu->qs_bytes(), b->stream) == hipSuccess;
+    ok = ok && launch_zero(F, cu->f_bytes(), b->stream) == hipSuccess &&
+         launch_zero(Nu, cu->nu_bytes(), b->stream) == hipSuccess &&
+         launch_zero(fm, cu->mask_bytes(), b->stream) == hipSuccess;
+    for (int32_t tile : {(int32_t)TILE_ROW64, (int32_t)TILE_PLANE}) {
+        if (!ok) break;
+        MemberDev md{};
+        md.g = cu->g;
+        md.P = b->prec == TDLB_PREC_F32 ? make_collide_params<float, double>(1e-3f, 0.1f, 0.8f, 1)
+                                        : make_collide_params<float, float>(1e-3f, 0.1f, 0.8f, 1);
+        md.rho0 = 1.0f;
+        md.tile = tile;
+        md.Q[0] = q[0];
+        md.Q[1] = q[nq - 1];
+        md.F = static_cast<float*>(F);
+        md.Nu = static_cast<float*>(Nu);
+        md.fmask[0] = md.fmask[1] = static_cast<uint64_t*>(fm);
+        md.active = 1;
+        md.n_steps = 1;
+        const int64_t tab[2] = {0, batch_coll_blocks(cu->g, tile)};
+        BatchPlan p{};
+        p.mem = static_cast<const MemberDev*>(dmem);
+        p.n_members = 1;
+        p.off_coll = static_cast<const int64_t*>(dtab);
+        p.tot_coll = tab[1];
+        ok = hipMemcpyAsync(dmem, &md, sizeof(md), hipMemcpyHostToDevice, b->stream) == hipSuccess &&
+             hipMemcpyAsync(dtab, tab, sizeof(tab), hipMemcpyHostToDevice, b->stream) == hipSuccess;
+        for (int w = 0; w < 2 && ok; w++)
+            ok = launch_batch_fused(b->prec, b->layout, b->streaming, p, b->stream) == hipSuccess;
+        ok = ok && hipEventRecord(e0, b->stream) == hipSuccess;
+        constexpr int reps = 5;
+        for (int w = 0; w < reps && ok; w++)
+            ok = launch_batch_fused(b->prec, b->layout, b->streaming, p, b->stream) == hipSuccess;
+        ok = ok && hipEventRecord(e1, b->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+        float ms = 0.f;
+        ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+        if (ok && ms < best) {
+            best = ms;
+            best_tile = tile;
+        }
+    }
+    if (!ok) rc = fail(TDLB_ERR_HIP, "autotune launch failed");
+    (void)hipStreamSynchronize(b->stream);
+    cleanup();
+    if (rc != TDLB_OK) return rc;
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    g_tiles[key] = best_tile;
+    *out = best_tile;
+    return TDLB_OK;
+}
+
+// A member needs one step on its own handle first when its state is not the
+// fused path's steady state (explicit moments, static stencils not in the mask,
+// O written by the caller).
+int settle(tdlb_batch* b, Member& m) {
+    tdlb_cu* cu = m.cu;
+    if (m.remaining <= 0) return TDLB_OK;
+    if (cu->phase == PH_LAZY && cu->static_in_prev && !cu->o_dirty) return TDLB_OK;
+    CHK(tdlb_cu_run(cu, 1, &m.step, &m.angle));
+    HIPCHK(hipStreamSynchronize(cu->stream));
+    m.remaining -= 1;
+    (void)b;
+    return TDLB_OK;
+}
+
+// Descriptors, prefix tables, level segments and blade tables of a run of K steps.
+int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
+    std::vector<int> idx;
+    for (size_t a = 0; a < b->members.size(); a++)
+        if (b->members[a].call > 0) idx.push_back((int)a);
+    const int M = (int)idx.size();
+    std::vector<std::pair<tdlb_cu*, int32_t>> key;
+    for (int a : idx) key.push_back({b->members[a].cu, b->members[a].tile});
+
+    std::vector<MemberDev> mem(M);
+    std::vector<int64_t> tab(4 * (size_t)(M + 1), 0);
+    int64_t* off_coll = tab.data();
+    int64_t* off_rot = off_coll + (M + 1);
+    int64_t* off_words = off_rot + (M + 1);
+    int64_t* off_scells = off_words + (M + 1);
+    std::vector<BladeStep> steps;
+    std::vector<size_t> step_base(M);
+    for (int r = 0; r < M; r++) {
+        Member& m = b->members[idx[r]];
+        tdlb_cu* cu = m.cu;
+        m.n_steps = std::min<int64_t>(m.call, K);
+        off_coll[r + 1] = off_coll[r] + batch_coll_blocks(cu->g, m.tile);
+        off_rot[r + 1] = off_rot[r] + cu->n_rot;
+        off_words[r + 1] = off_words[r] + (int64_t)cu->mask_words;
+        off_scells[r + 1] = off_scells[r] + cu->static_set.n_cells;
+        // blade orientation of each step (the single-handle loop computes the same)
+        Rushton R(cu->rp);
+        m.angles.resize(m.n_steps);
+        const size_t base = steps.size();
+        step_base[r] = base;
+        steps.resize(base + m.n_steps);
+        double angle = m.angle;
+        for (int64_t s = 0; s < m.n_steps; s++) {
+            const double delta = to_prec_f(cu, R.step_increment(m.step + 1 + s));
+            angle += delta;
+            m.angles[s] = angle;
+            BladeStep& st = steps[base + s];
+            std::memset(&st, 0, sizeof(st));
+            R.blade_trig(angle, st.cosb, st.sinb);
+            st.omega = delta;
+        }
+        mem[r] = describe(b, m);
+        mem[r].n_steps = m.n_steps;
+        mem[r].active = m.n_steps > 0 ? 1 : 0;
+        mem[r].done = 0;
+    }
+    // forcing levels: level l of every member that has it
+    std::vector<LevelSeg> segs;
+    std::vector<int64_t> cum;
+    b->lv_static.assign(1, 0);
+    b->lv_blade.assign(1, 0);
+    b->lv_threads.clear();
+    auto add_levels = [&](bool stat, std::vector<int64_t>& lv) {
+        size_t nl = 0;
+        for (int r = 0; r < M; r++) {
+            const tdlb_cu* cu = b->members[idx[r]].cu;
+            const auto& L = stat ? cu->static_levels : cu->rot_levels;
+            if (L.size() > 1) nl = std::max(nl, L.size() - 1);
+        }
+        for (size_t l = 0; l < nl; l++) {
+            int64_t threads = 0;
+            for (int r = 0; r < M; r++) {
+                const tdlb_cu* cu = b->members[idx[r]].cu;
+                const auto& L = stat ? cu->static_levels : cu->rot_levels;
+                if (l + 1 >= L.size()) continue;
+                const int64_t cnt = L[l + 1] - L[l];
+                if (cnt <= 0) continue;
+                segs.push_back(LevelSeg{r, 0, L[l], cnt});
+                cum.push_back(threads);
+                threads += cnt;
+            }
+            lv.push_back((int64_t)segs.size());
+            b->lv_threads.push_back(threads);
+        }
+    };
+    add_levels(true, b->lv_static);
+    add_levels(false, b->lv_blade);
+    const size_t ntab = tab.size() + cum.size();
+
+    int moved = 0;
+    int g;
+    if ((g = b->d_mem.grow(std::max(M, 1))) < 0) return fail(TDLB_ERR_NOMEM, "batch descriptors");
+    moved |= g;
+    if ((g = b->d_tab.grow(ntab)) < 0) return fail(TDLB_ERR_NOMEM, "batch tables");
+    moved |= g;
+    if ((g = b->d_segs.grow(std::max<size_t>(segs.size(), 1))) < 0) return fail(TDLB_ERR_NOMEM, "batch levels");
+    moved |= g;
+    if (b->d_steps.grow(std::max<size_t>(steps.size(), 1)) < 0) return fail(TDLB_ERR_NOMEM, "batch blade table");
+    for (int r = 0; r < M; r++) mem[r].steps = b->d_steps.p + step_base[r];
+    tab.insert(tab.end(), cum.begin(), cum.end());
+    if (M > 0) HIPCHK(hipMemcpyAsync(b->d_mem.p, mem.data(), M * sizeof(MemberDev), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->d_tab.p, tab.data(), tab.size() * sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
+    if (!segs.empty())
+        HIPCHK(hipMemcpyAsync(b->d_segs.p, segs.data(), segs.size() * sizeof(LevelSeg), hipMemcpyHostToDevice,
+                              b->stream));
+    if (!steps.empty())
+        HIPCHK(hipMemcpyAsync(b->d_steps.p, steps.data(), steps.size() * sizeof(BladeStep), hipMemcpyHostToDevice,
+                              b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));  // host vectors go out of scope
+
+    BatchPlan& p = b->plan;
+    p.mem = b->d_mem.p;
+    p.mem_rw = b->d_mem.p;
+    p.n_members = M;
+    p.off_coll = b->d_tab.p;
+    p.off_rot = b->d_tab.p + (M + 1);
+    p.off_words = b->d_tab.p + 2 * (M + 1);
+    p.off_scells = b->d_tab.p + 3 * (M + 1);
+    p.seg_cum = b->d_tab.p + 4 * (M + 1);
+    p.segs = b->d_segs.p;
+    p.tot_coll = off_coll[M];
+    p.tot_rot = off_rot[M];
+    p.tot_words = off_words[M];
+    p.tot_scells = off_scells[M];
+    p.n_slevels = (int32_t)b->lv_static.size() - 1;
+    p.n_rlevels = (int32_t)b->lv_blade.size() - 1;
+    p.lv_static = b->lv_static.data();
+    p.lv_blade = b->lv_blade.data();
+    p.lv_threads = b->lv_threads.data();
+    // launches: blades, fused, mask, prepare, static moments, blade moments, levels, advance
+    b->launches_per_step = 1 + 1 + 1 + 1 + 1 + 1 + p.n_slevels + p.n_rlevels + 1;
+    *structure_changed = moved || key != b->key || b->buffers_moved;
+    b->buffers_moved = false;
+    b->key = key;
+    b->active_idx = idx;
+    return TDLB_OK;
+}
+
+int record_step(tdlb_batch* b) {
+    HIPCHK(launch_batch_step(b->prec, b->layout, b->streaming, b->plan, b->stream));
+    return TDLB_OK;
+}
+
+int capture(tdlb_batch* b) {
+    drop_graph(b);
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int r = record_step(b);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (r != TDLB_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    return TDLB_OK;
+}
+
+Member* find_member(tdlb_batch* b, tdlb_cu* cu) {
+    for (auto& m : b->members)
+        if (m.cu == cu) return &m;
+    return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tdlb_batch_create(int32_t precision, int32_t layout, int32_t streaming, int32_t device_id, tdlb_batch_t* out) {
+    if (!out) return fail(TDLB_ERR_INVALID, "null argument");
+    if (precision != TDLB_PREC_F32 && precision != TDLB_PREC_F16 && precision != TDLB_PREC_BF16)
+        return fail(TDLB_ERR_UNSUPPORTED, "batches run the float-arithmetic precisions (fp32, fp16, bf16)");
+    if (layout != TDLB_LAYOUT_IJKL && layout != TDLB_LAYOUT_LIJK) return fail(TDLB_ERR_INVALID, "layout");
+    if (streaming != TDLB_STREAM_SIMPLE && streaming != TDLB_STREAM_ESOTWIST) return fail(TDLB_ERR_INVALID, "streaming");
+    tdlb_batch* b = new tdlb_batch();
+    b->prec = precision;
+    b->layout = layout;
+    b->streaming = streaming;
+    b->device = device_id < 0 ? 0 : device_id;
+    if (hipSetDevice(b->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return fail(TDLB_ERR_HIP, "batch stream");
+    }
+    *out = b;
+    return TDLB_OK;
+}
+
+int tdlb_batch_destroy(tdlb_batch_t b) {
+    if (!b) return TDLB_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    b->d_mem.release();
+    b->d_tab.release();
+    b->d_segs.release();
+    b->d_steps.release();
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return TDLB_OK;
+}
+
+int tdlb_batch_set_options(tdlb_batch_t b, int32_t use_graph, int32_t autotune) {
+    if (!b) return fail(TDLB_ERR_INVALID, "null batch");
+    b->use_graph = use_graph != 0;
+    b->autotune = autotune != 0;
+    b->buffers_moved = true;  // recapture on the next run
+    return TDLB_OK;
+}
+
+int tdlb_batch_add(tdlb_batch_t b, tdlb_cu_t cu, uint64_t step, double angle, int64_t nsteps) {
+    if (!b || !cu || nsteps < 0) return fail(TDLB_ERR_INVALID, "bad arguments");
+    if (cu->prm.precision != b->prec || cu->prm.layout != b->layout || cu->prm.streaming != b->streaming)
+        return fail(TDLB_ERR_INVALID, "member precision, layout and streaming must match the batch");
+    if (cu->device != b->device) return fail(TDLB_ERR_INVALID, "member is on another device");
+    if (!cu->bound) return fail(TDLB_ERR_STATE, "member has no bound geometry (tdlb_cu_bind_rushton)");
+    if (cu->g.xg1 < 3 || cu->g.zg1 < 3) return fail(TDLB_ERR_UNSUPPORTED, "members need at least 3 cells in x and z");
+    if (find_member(b, cu)) return fail(TDLB_ERR_INVALID, "handle is already in the batch");
+    HIPCHK(hipSetDevice(cu->device));
+    Member m;
+    m.cu = cu;
+    m.step = step;
+    m.angle = angle;
+    m.remaining = nsteps;
+    CHK(tune_tile(b, cu, &m.tile));
+    b->members.push_back(m);
+    return TDLB_OK;
+}
+
+int tdlb_batch_remove(tdlb_batch_t b, tdlb_cu_t cu) {
+    if (!b || !cu) return fail(TDLB_ERR_INVALID, "null argument");
+    for (size_t a = 0; a < b->members.size(); a++)
+        if (b->members[a].cu == cu) {
+            b->members.erase(b->members.begin() + (long)a);
+            return TDLB_OK;
+        }
+    return fail(TDLB_ERR_INVALID, "handle is not in the batch");
+}
+
+int tdlb_batch_member(tdlb_batch_t b, tdlb_cu_t cu, uint64_t* step, double* angle, int64_t* remaining) {
+    if (!b || !cu) return fail(TDLB_ERR_INVALID, "null argument");
+    Member* m = find_member(b, cu);
+    if (!m) return fail(TDLB_ERR_INVALID, "handle is not in the batch");
+    if (step) *step = m->step;
+    if (angle) *angle = m->angle;
+    if (remaining) *remaining = m->remaining;
+    return TDLB_OK;
+}
+
+int tdlb_batch_run(tdlb_batch_t b, int64_t nsteps, int64_t* steps_run) {
+    if (!b || nsteps < 0) return fail(TDLB_ERR_INVALID, "bad arguments");
+    HIPCHK(hipSetDevice(b->device));
+    if (steps_run) *steps_run = 0;
+    if (nsteps == 0) return TDLB_OK;
+    // every member takes min(nsteps, remaining) steps; one whose state is not
+    // the steady fused state takes the first of them alone
+    int64_t most = 0, K = 0;
+    for (auto& m : b->members) {
+        HIPCHK(hipStreamSynchronize(m.cu->stream));
+        m.call = std::min<int64_t>(m.remaining, nsteps);
+        most = std::max(most, m.call);
+        const int64_t before = m.remaining;
+        if (m.call > 0) CHK(settle(b, m));
+        m.call -= before - m.remaining;
+        K = std::max(K, m.call);
+    }
+    if (steps_run) *steps_run = most;
+    if (K <= 0) return TDLB_OK;
+    bool changed = false;
+    CHK(build_plan(b, K, &changed));
+    if (b->plan.n_members == 0) return TDLB_OK;
+    if (b->use_graph) {
+        if (changed || !b->exec) CHK(capture(b));
+        for (int64_t s = 0; s < K; s++) HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    } else {
+        for (int64_t s = 0; s < K; s++) CHK(record_step(b));
+    }
+    std::vector<MemberDev> mem(b->plan.n_members);
+    HIPCHK(hipMemcpyAsync(mem.data(), b->d_mem.p, mem.size() * sizeof(MemberDev), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (size_t r = 0; r < mem.size(); r++) {
+        Member& m = b->members[b->active_idx[r]];
+        tdlb_cu* cu = m.cu;
+        const int64_t done = mem[r].done;
+        cu->even = mem[r].even != 0;
+        cu->cur = mem[r].cur;
+        cu->fm = mem[r].fm;
+        if (done > 0) {
+            m.step += (uint64_t)done;
+            m.angle = m.angles[done - 1];
+            m.remaining -= done;
+        }
+        m.call = 0;
+    }
+    return TDLB_OK;
+}
+
+int tdlb_batch_info(tdlb_batch_t b, int64_t* n_members, int64_t* launches_per_step, int32_t* tiles, int64_t cap) {
+    if (!b) return fail(TDLB_ERR_INVALID, "null batch");
+    if (n_members) *n_members = (int64_t)b->members.size();
+    if (launches_per_step) *launches_per_step = b->launches_per_step;
+    if (tiles)
+        for (int64_t a = 0; a < cap && a < (int64_t)b->members.size(); a++) tiles[a] = b->members[a].tile;
+    return TDLB_OK;
+}
+
+}  // extern "C"

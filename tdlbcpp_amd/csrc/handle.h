@@ -1,0 +1,173 @@
+// State of a ComputeUnit handle (private to the library: capi.cpp, batch.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tdlb.h"
+#include "kernels.h"
+
+namespace tdlb {
+namespace detail {
+
+// records the message for tdlb_last_error() and returns code
+int fail(int code, const std::string& msg);
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(TDLB_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+#define CHK(expr)                  \
+    do {                           \
+        int r_ = (expr);           \
+        if (r_ != TDLB_OK) return r_; \
+    } while (0)
+
+enum Phase { PH_EXPLICIT = 0, PH_LAZY = 1 };
+
+struct DevPoints {
+    PointsDev d{};
+    int64_t cap = 0;
+    size_t tsize = 4;
+    void release() {
+        void* ptrs[] = {d.i, d.j, d.k, d.ifrac, d.kfrac, d.u, d.v, d.w};
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+        d = PointsDev{};
+        cap = 0;
+    }
+    int reserve(int64_t n, size_t ts) {
+        if (n <= cap && ts == tsize) return TDLB_OK;
+        release();
+        tsize = ts;
+        const int64_t c = std::max<int64_t>(n, 256);
+        void** ip[] = {(void**)&d.i, (void**)&d.j, (void**)&d.k};
+        for (void** p : ip)
+            if (hipMalloc(p, sizeof(int32_t) * c) != hipSuccess) return fail(TDLB_ERR_NOMEM, "IB point buffers");
+        void** fp[] = {&d.ifrac, &d.kfrac, &d.u, &d.v, &d.w};
+        for (void** p : fp)
+            if (hipMalloc(p, ts * c) != hipSuccess) return fail(TDLB_ERR_NOMEM, "IB point buffers");
+        cap = c;
+        return TDLB_OK;
+    }
+};
+
+// Stencil cells of a point list with a moments cache: the sorted unique cells,
+// and for every point the 9 positions of its stencil cells in that list.
+struct IndexedSet {
+    uint32_t* cells = nullptr;
+    int32_t* sidx = nullptr;
+    void* cache = nullptr;
+    int64_t n_cells = 0, n_points = 0;
+    void release() {
+        if (cells) (void)hipFree(cells);
+        if (sidx) (void)hipFree(sidx);
+        if (cache) (void)hipFree(cache);
+        cells = nullptr;
+        sidx = nullptr;
+        cache = nullptr;
+        n_cells = n_points = 0;
+    }
+};
+
+// Host-side copy of points in the CU's type, structure of arrays.
+struct HostPoints {
+    std::vector<int32_t> i, j, k;
+    std::vector<uint8_t> ifrac, kfrac, u, v, w;
+    void resize(size_t n, size_t ts) {
+        i.resize(n); j.resize(n); k.resize(n);
+        ifrac.resize(n * ts); kfrac.resize(n * ts); u.resize(n * ts); v.resize(n * ts); w.resize(n * ts);
+    }
+};
+
+template <typename T>
+void put(std::vector<uint8_t>& v, size_t idx, double x) {
+    T t = (T)x;
+    std::memcpy(v.data() + idx * sizeof(T), &t, sizeof(T));
+}
+
+}  // namespace detail
+}  // namespace tdlb
+
+using namespace tdlb;
+using namespace tdlb::detail;
+
+struct tdlb_cu {
+    tdlb_cu_params prm{};
+    tdlb_flow flow{};
+    Grid g{};
+    int64_t xg = 0, yg = 0, zg = 0;
+    size_t tsize = 4;  // arithmetic type: F, Nu, points, explicit Q
+    size_t ssize = 4;  // stored population type
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    void* Q[2] = {nullptr, nullptr};
+    int cur = 0;
+    void* M = nullptr;   // explicit Q of the reduced-precision storage (float)
+    float rho0 = 1.0f;   // rest density of the deviation storage
+    double* red = nullptr;  // FP64 partial sums + result of tdlb_cu_reductions
+    void* F = nullptr;
+    void* Nu = nullptr;
+    uint8_t* O = nullptr;
+    uint64_t* fmask[2] = {nullptr, nullptr};
+    uint64_t* gmask = nullptr;
+    uint64_t* static_bits = nullptr;
+    uint64_t* static_base = nullptr;
+    int fm = 0;
+    size_t mask_words = 0;
+
+    bool even = true;
+    Phase phase = PH_EXPLICIT;
+    bool o_dirty = false;         // O may hold non-zero interior values
+    bool static_in_prev = false;  // every static stencil cell is set in fmask[fm]
+
+    // bound Rushton tank
+    bool bound = false;
+    tdlb_rushton_params rp{};
+    DevPoints pts;                       // static points [0, n_static), blades after
+    IndexedSet static_set;               // stencil cells of the static points
+    void* rot_cache = nullptr;           // 9 moment entries per blade point
+    int64_t n_static = 0, n_rot = 0;
+    std::vector<int64_t> static_levels;  // CSR offsets into [0, n_static)
+    std::vector<int64_t> rot_levels;     // CSR offsets into [n_static, n_static + n_rot)
+    BladeLayout blade{};
+    double alfa = 0.97, beta = 1.9;
+
+    DevPoints api_pts;  // points of the generic forcing call
+    IndexedSet api_set;
+
+    // event timing of tdlb_cu_run (tdlb_cu_profile)
+    bool prof = false;
+    std::vector<hipEvent_t> events;        // pool, recorded in pairs
+    std::vector<std::pair<int, int>> coll, force;
+    size_t ev_used = 0;
+
+    size_t q_bytes() const { return (size_t)g.size * kQ * tsize; }   // Q as the reference holds it
+    size_t qs_bytes() const { return (size_t)g.size * kQ * ssize; }  // one population lattice
+    size_t f_bytes() const { return (size_t)g.size * 3 * tsize; }
+    size_t nu_bytes() const { return (size_t)g.size * tsize; }
+    size_t mask_bytes() const { return mask_words * sizeof(uint64_t); }
+    bool simple() const { return prm.streaming == TDLB_STREAM_SIMPLE; }
+    bool shifted() const { return prm.precision == TDLB_PREC_F16 || prm.precision == TDLB_PREC_BF16; }
+    int prec() const { return prm.precision; }            // kernels on the stored populations
+    int eprec() const { return prm.precision == TDLB_PREC_F64 ? 1 : 0; }  // kernels on the explicit Q
+    void* eq() const { return shifted() ? M : Q[cur]; }   // the explicit Q array
+
+    CollideArgs cargs() const {
+        CollideArgs c;
+        c.nu = flow.nu;
+        c.cs0 = flow.cs0;
+        c.g3 = flow.g3;
+        c.use_les = flow.use_les;
+        return c;
+    }
+};

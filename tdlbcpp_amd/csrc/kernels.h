@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include "device.h"
+#include "es_collide.h"
 
 namespace tdlb {
 
@@ -127,5 +128,78 @@ hipError_t launch_gen_blades(int prec, const BladeLayout& bl, const BladeStep& s
                              int64_t offset, hipStream_t s);
 hipError_t launch_solid_mask(double cx, double cz, double radius, uint8_t* out, const Grid& g, int ghost,
                              hipStream_t s);
+
+// ---------------------------------------------------------------- batches of ComputeUnits
+// Many bound ComputeUnits (float arithmetic: prec 0, 2, 3; one streaming and
+// layout) advance together: every kernel of a main-loop step is one launch over
+// all members, blocks mapped to members through prefix tables.  Per-member
+// state that changes from step to step (parity, lattice, occupancy mask, step
+// index) lives in MemberDev on the device and is advanced by a kernel, so one
+// step is a fixed launch sequence that can be captured in a HIP graph.
+enum Tile : int32_t {
+    TILE_ROW64 = 0,  // a wavefront takes 64 consecutive k of one (i, j) row
+    TILE_PLANE = 1,  // a block takes 256 consecutive cells of one i plane (ghost cells idle)
+};
+
+struct MemberDev {
+    Grid g;
+    CollideParams<float> P;
+    float rho0;
+    float alfa, beta;
+    int32_t tile;
+    void* Q[2];  // Esotwist: Q[0]; Simple: the two lattices
+    float* F;
+    float* Nu;
+    uint64_t* fmask[2];
+    const uint64_t* static_base;
+    PointsDev pts;  // static points [0, n_static), blades [n_static, n_static + n_rot)
+    int64_t n_static, n_rot;
+    const uint32_t* scells;  // unique stencil cells of the static points
+    const int32_t* sidx;     // 9 per static point into scells / scache
+    int64_t n_scells;
+    void* scache;  // Mom4<float> per unique static cell
+    void* rcache;  // Mom4<float> x 9 per blade point
+    BladeLayout blade;
+    const BladeStep* steps;  // blade orientation of step done + 1, done + 2, ...
+    int64_t n_steps;         // steps this member may still take in the batch
+    // advanced on the device after every step
+    int32_t even, cur, fm, active;
+    int64_t done;
+};
+
+// points [begin, begin + count) of member `member` in one forcing level
+struct LevelSeg {
+    int32_t member, pad;
+    int64_t begin, count;
+};
+
+// Device tables of one batch step.  off_* are member prefix sums (n_members + 1
+// entries) of blocks / threads; level l of the static (blade) lists is
+// segs[lv_*[l] .. lv_*[l + 1]) with thread prefix sums in seg_cum.
+struct BatchPlan {
+    const MemberDev* mem;
+    MemberDev* mem_rw;
+    int32_t n_members;
+    const int64_t* off_coll;    // blocks of the fused kernel
+    const int64_t* off_rot;     // blade points
+    const int64_t* off_words;   // occupancy-mask words
+    const int64_t* off_scells;  // unique static stencil cells
+    const LevelSeg* segs;
+    const int64_t* seg_cum;
+    int64_t tot_coll, tot_rot, tot_words, tot_scells;
+    // host copies of the level structure (launch sizes)
+    int32_t n_slevels, n_rlevels;
+    const int64_t* lv_static;   // host: n_slevels + 1 segment offsets
+    const int64_t* lv_blade;    // host: n_rlevels + 1 segment offsets
+    const int64_t* lv_threads;  // host: threads of static levels then blade levels
+};
+
+// Enqueue one main-loop step of every active member (fused stream-collide,
+// blade generation, occupancy masks, moment caches, forcing levels, advance).
+hipError_t launch_batch_step(int prec, int layout, int streaming, const BatchPlan& plan, hipStream_t s);
+// blocks of the fused kernel for one member with the given tile
+int64_t batch_coll_blocks(const Grid& g, int tile);
+// The fused kernel alone over every member (autotuning; advances no state).
+hipError_t launch_batch_fused(int prec, int layout, int streaming, const BatchPlan& plan, hipStream_t s);
 
 }  // namespace tdlb

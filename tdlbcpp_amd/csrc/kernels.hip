@@ -517,11 +517,9 @@ __device__ __forceinline__ int wrap_ik(int a, int a1) {  // Forcing.hpp:85-88
 // previous pass is zero in the reference (F=0 where O==0), so clear it here,
 // and mark the cell in the new occupancy mask.
 template <typename T>
-__global__ void k_force_prepare(PointsDev p, int64_t begin, int64_t count, T* __restrict__ F,
-                                const uint64_t* __restrict__ prev, uint64_t* __restrict__ next, Grid g) {
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    const int64_t q = begin + t;
+__device__ __forceinline__ void prepare_point(const PointsDev& p, int64_t q, T* __restrict__ F,
+                                              const uint64_t* __restrict__ prev, uint64_t* __restrict__ next,
+                                              const Grid& g) {
     const int pi = p.i[q], pj = p.j[q], pk = p.k[q];
     for (int k1 = -1; k1 <= 1; k1++)
         for (int i1 = -1; i1 <= 1; i1++) {
@@ -534,6 +532,14 @@ __global__ void k_force_prepare(PointsDev p, int64_t begin, int64_t count, T* __
             }
             atomicOr((unsigned long long*)&next[mask_word_index(g, i2, pj, k2)], 1ull << (k2 & 63));
         }
+}
+
+template <typename T>
+__global__ void k_force_prepare(PointsDev p, int64_t begin, int64_t count, T* __restrict__ F,
+                                const uint64_t* __restrict__ prev, uint64_t* __restrict__ next, Grid g) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    prepare_point(p, begin + t, F, prev, next, g);
 }
 
 // density and momentum of cell (i,j,k) as the reference's moments() would leave them
@@ -690,13 +696,9 @@ __global__ __launch_bounds__(kBlock) void k_ib_moments_points(PointsDev p, int64
 // Same arithmetic as k_force_level; stencil cell s = (k1+1)*3 + (i1+1) reads
 // cache[sidx[9q+s]] (INDEXED) or cache[9(q-cache_base)+s].
 template <typename T, typename D, bool INDEXED>
-__global__ __launch_bounds__(kBlock) void k_force_level_cached(PointsDev p, int64_t begin, int64_t count,
-                                                               const int32_t* __restrict__ sidx, int64_t cache_base,
-                                                               const Mom4<T>* __restrict__ cache, T* __restrict__ F,
-                                                               Grid g, T alfa, T beta) {
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    const int64_t q = begin + t;
+__device__ __forceinline__ void force_point_cached(const PointsDev& p, int64_t q, const int32_t* __restrict__ sidx,
+                                                   int64_t cache_base, const Mom4<T>* __restrict__ cache,
+                                                   T* __restrict__ F, const Grid& g, T alfa, T beta) {
     const int pi = p.i[q], pj = p.j[q], pk = p.k[q];
     const T ifr = static_cast<const T*>(p.ifrac)[q], kfr = static_cast<const T*>(p.kfrac)[q];
     T wx[3], wz[3];
@@ -740,6 +742,16 @@ __global__ __launch_bounds__(kBlock) void k_force_level_cached(PointsDev p, int6
         F[3 * cell[s] + 1] = alfa * fo[s][1] - bp * ySum;
         F[3 * cell[s] + 2] = alfa * fo[s][2] - bp * zSum;
     }
+}
+
+template <typename T, typename D, bool INDEXED>
+__global__ __launch_bounds__(kBlock) void k_force_level_cached(PointsDev p, int64_t begin, int64_t count,
+                                                               const int32_t* __restrict__ sidx, int64_t cache_base,
+                                                               const Mom4<T>* __restrict__ cache, T* __restrict__ F,
+                                                               Grid g, T alfa, T beta) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    force_point_cached<T, D, INDEXED>(p, begin + t, sidx, cache_base, cache, F, g, alfa, beta);
 }
 
 // ------------------------------------------------------------------ FP64 sums and macroscopic fields
@@ -842,9 +854,8 @@ __global__ __launch_bounds__(kBlock) void k_macroscopic(const typename PR::S* __
 
 // ------------------------------------------------------------------ geometry on the device
 template <typename T>
-__global__ void k_gen_blades(BladeLayout bl, BladeStep st, PointsDev out, int64_t offset, int64_t count) {
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= count) return;
+__device__ __forceinline__ void blade_gen_point(const BladeLayout& bl, const BladeStep& st, const PointsDev& out,
+                                                int64_t offset, int64_t t) {
     BladeEnum e;
     e.n_blades = bl.n_blades;
     e.n_planes = bl.n_planes;
@@ -863,6 +874,13 @@ __global__ void k_gen_blades(BladeLayout bl, BladeStep st, PointsDev out, int64_
     static_cast<T*>(out.u)[q] = (T)pt.u_delta;
     static_cast<T*>(out.v)[q] = (T)pt.v_delta;
     static_cast<T*>(out.w)[q] = (T)pt.w_delta;
+}
+
+template <typename T>
+__global__ void k_gen_blades(BladeLayout bl, BladeStep st, PointsDev out, int64_t offset, int64_t count) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    blade_gen_point<T>(bl, st, out, offset, t);
 }
 
 // getExternalPoints as a mask: a cell is outside when its centre lies beyond the
@@ -1302,6 +1320,275 @@ hipError_t launch_solid_mask(double cx, double cz, double radius, uint8_t* out, 
                              hipStream_t s) {
     hipLaunchKernelGGL(k_solid_mask, dim3(grid_blocks(g.size)), dim3(kBlock), 0, s, cx, cz, radius, out, g, ghost);
     return hipGetLastError();
+}
+
+// ====================================================================== batches
+// Each kernel below runs the per-member body of the single-handle kernel above
+// (same device functions), so a member advances exactly as it would alone.
+namespace {
+
+// largest m in [0, n) with off[m] <= x (off non-decreasing, off[0] <= x)
+__device__ __forceinline__ int find_slot(const int64_t* __restrict__ off, int n, int64_t x) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// interior cell of thread threadIdx.x in block b of a member's fused launch
+__device__ __forceinline__ bool batch_cell(const Grid& g, int tile, uint32_t b, int& i, int& j, int& k) {
+    if (tile == TILE_ROW64) {
+        const uint32_t segs = (uint32_t)(g.zg1 + 63) >> 6;
+        const uint32_t w = b * (kBlock / 64) + (threadIdx.x >> 6);
+        const uint32_t rows = (uint32_t)g.xg1 * (uint32_t)g.yg1;
+        if (w >= rows * segs) return false;
+        const uint32_t row = w / segs, seg = w - row * segs;
+        const uint32_t ii = row / (uint32_t)g.yg1;
+        i = (int)ii + 1;
+        j = (int)(row - ii * (uint32_t)g.yg1) + 1;
+        k = (int)(seg * 64 + (threadIdx.x & 63)) + 1;
+        return k <= g.zg1;
+    }
+    const uint32_t plane = (uint32_t)g.yg * (uint32_t)g.zg;
+    const uint32_t bpp = (plane + kBlock - 1) / kBlock;
+    const uint32_t ip = b / bpp;
+    const uint32_t local = (b - ip * bpp) * kBlock + threadIdx.x;
+    if (local >= plane) return false;
+    i = (int)ip + 1;
+    j = (int)(local / (uint32_t)g.zg);
+    k = (int)(local - (uint32_t)j * (uint32_t)g.zg);
+    return j >= 1 && j <= g.yg1 && k >= 1 && k <= g.zg1;
+}
+
+template <typename PR, int L, int STREAM>
+__global__ __launch_bounds__(kBlock) void k_batch_fused(const MemberDev* __restrict__ mem,
+                                                        const int64_t* __restrict__ off, int n) {
+    static_assert(std::is_same<typename PR::T, float>::value, "batches compute in float");
+    using S = typename PR::S;
+    const int m = find_slot(off, n, blockIdx.x);
+    const MemberDev& md = mem[m];
+    if (!md.active) return;
+    int i, j, k;
+    if (!batch_cell(md.g, md.tile, (uint32_t)(blockIdx.x - off[m]), i, j, k)) return;
+    const uint64_t* fm = md.fmask[md.fm];
+    if constexpr (STREAM == StreamEsotwist) {
+        S* Q = static_cast<S*>(md.Q[0]);
+        if (md.even) esotwist_cell<PR, L, MODE_FUSED, 1>(Q, nullptr, md.F, md.Nu, fm, md.g, md.P, md.rho0, i, j, k);
+        else esotwist_cell<PR, L, MODE_FUSED, 0>(Q, nullptr, md.F, md.Nu, fm, md.g, md.P, md.rho0, i, j, k);
+    } else {
+        simple_cell<PR, L, MODE_FUSED>(static_cast<const S*>(md.Q[md.cur]), static_cast<S*>(md.Q[1 - md.cur]),
+                                       nullptr, md.F, md.Nu, fm, md.g, md.P, md.rho0, i, j, k);
+    }
+}
+
+__global__ void k_batch_blades(const MemberDev* __restrict__ mem, const int64_t* __restrict__ off, int n,
+                               int64_t total) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int m = find_slot(off, n, t);
+    const MemberDev& md = mem[m];
+    if (!md.active) return;
+    blade_gen_point<float>(md.blade, md.steps[md.done], md.pts, md.n_static, t - off[m]);
+}
+
+// the new occupancy mask starts from the static stencil bits
+__global__ void k_batch_mask_init(const MemberDev* __restrict__ mem, const int64_t* __restrict__ off, int n,
+                                  int64_t total) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int m = find_slot(off, n, t);
+    const MemberDev& md = mem[m];
+    if (!md.active) return;
+    md.fmask[1 - md.fm][t - off[m]] = md.static_base[t - off[m]];
+}
+
+__global__ void k_batch_prepare(const MemberDev* __restrict__ mem, const int64_t* __restrict__ off, int n,
+                                int64_t total) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int m = find_slot(off, n, t);
+    const MemberDev& md = mem[m];
+    if (!md.active) return;
+    prepare_point(md.pts, md.n_static + (t - off[m]), md.F, md.fmask[md.fm], md.fmask[1 - md.fm], md.g);
+}
+
+// density and momentum of a cell from the populations the fused kernel just wrote
+template <typename PR, int L, int STREAM>
+__device__ __forceinline__ void batch_rho_j(const MemberDev& md, int i, int j, int k, Mom4<float>& m) {
+    using S = typename PR::S;
+    if constexpr (STREAM == StreamEsotwist) {
+        const S* Q = static_cast<const S*>(md.Q[0]);
+        if (md.even) cell_rho_j<PR, L, STREAM, MOMSRC_ESO_POPS, 0>(Q, md.g, i, j, k, m.r, m.x, m.y, m.z, md.rho0);
+        else cell_rho_j<PR, L, STREAM, MOMSRC_ESO_POPS, 1>(Q, md.g, i, j, k, m.r, m.x, m.y, m.z, md.rho0);
+    } else {
+        const S* Q = static_cast<const S*>(md.Q[1 - md.cur]);
+        cell_rho_j<PR, L, STREAM, MOMSRC_PULL_POPS, 1>(Q, md.g, i, j, k, m.r, m.x, m.y, m.z, md.rho0);
+    }
+}
+
+template <typename PR, int L, int STREAM>
+__global__ __launch_bounds__(kBlock) void k_batch_moments_static(const MemberDev* __restrict__ mem,
+                                                                 const int64_t* __restrict__ off, int n,
+                                                                 int64_t total) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int m = find_slot(off, n, t);
+    const MemberDev& md = mem[m];
+    if (!md.active) return;
+    const int64_t e = t - off[m];
+    const uint32_t c = md.scells[e];
+    const int i = (int)(c / md.g.yz);
+    const uint32_t r = c - (uint32_t)i * md.g.yz;
+    const int j = (int)(r / (uint32_t)md.g.zg);
+    const int k = (int)(r - (uint32_t)j * (uint32_t)md.g.zg);
+    Mom4<float> mm;
+    batch_rho_j<PR, L, STREAM>(md, i, j, k, mm);
+    static_cast<Mom4<float>*>(md.scache)[e] = mm;
+}
+
+// off counts blade points; thread t handles stencil cell t % 9 of point t / 9
+template <typename PR, int L, int STREAM>
+__global__ __launch_bounds__(kBlock) void k_batch_moments_blades(const MemberDev* __restrict__ mem,
+                                                                 const int64_t* __restrict__ off, int n,
+                                                                 int64_t total9) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= total9) return;
+    const int m = find_slot(off, n, t / 9);
+    const MemberDev& md = mem[m];
+    if (!md.active) return;
+    const int64_t e = t - 9 * off[m];
+    const int64_t q = md.n_static + e / 9;
+    const int s = (int)(e % 9);
+    const int i2 = wrap_ik(md.pts.i[q] + (s % 3) - 1, md.g.xg1), k2 = wrap_ik(md.pts.k[q] + (s / 3) - 1, md.g.zg1);
+    Mom4<float> mm;
+    batch_rho_j<PR, L, STREAM>(md, i2, md.pts.j[q], k2, mm);
+    static_cast<Mom4<float>*>(md.rcache)[e] = mm;
+}
+
+// one forcing level over the segments [sb, se) (static lists INDEXED, blades direct)
+template <bool INDEXED>
+__global__ __launch_bounds__(kBlock) void k_batch_force_level(const MemberDev* __restrict__ mem,
+                                                              const LevelSeg* __restrict__ segs,
+                                                              const int64_t* __restrict__ cum, int64_t sb, int ns,
+                                                              int64_t total) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int64_t s = sb + find_slot(cum + sb, ns, t);
+    const LevelSeg sg = segs[s];
+    const MemberDev& md = mem[sg.member];
+    if (!md.active) return;
+    const int64_t q = sg.begin + (t - cum[s]);
+    if (INDEXED)
+        force_point_cached<float, double, true>(md.pts, q, md.sidx, 0, static_cast<const Mom4<float>*>(md.scache),
+                                                md.F, md.g, md.alfa, md.beta);
+    else
+        force_point_cached<float, double, false>(md.pts, q, nullptr, md.n_static,
+                                                 static_cast<const Mom4<float>*>(md.rcache), md.F, md.g, md.alfa,
+                                                 md.beta);
+}
+
+template <int STREAM>
+__global__ void k_batch_advance(MemberDev* __restrict__ mem, int n) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    MemberDev& md = mem[m];
+    if (!md.active) return;
+    md.done += 1;
+    if (STREAM == StreamEsotwist) md.even ^= 1;
+    else md.cur ^= 1;
+    md.fm ^= 1;
+    if (md.done >= md.n_steps) md.active = 0;
+}
+
+inline uint32_t nblk(int64_t n, int bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+template <typename PR, int L, int STREAM>
+hipError_t batch_fused_t(const BatchPlan& p, hipStream_t s) {
+    if (p.tot_coll <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_batch_fused<PR, L, STREAM>), dim3((uint32_t)p.tot_coll), dim3(kBlock), 0, s, p.mem,
+                       p.off_coll, p.n_members);
+    return hipGetLastError();
+}
+
+template <typename PR, int L, int STREAM>
+hipError_t batch_step_t(const BatchPlan& p, hipStream_t s) {
+    const int n = p.n_members;
+    if (p.tot_rot > 0)
+        hipLaunchKernelGGL(k_batch_blades, dim3(nblk(p.tot_rot, kBlock)), dim3(kBlock), 0, s, p.mem, p.off_rot, n,
+                           p.tot_rot);
+    hipError_t e = batch_fused_t<PR, L, STREAM>(p, s);
+    if (e != hipSuccess) return e;
+    if (p.tot_words > 0)
+        hipLaunchKernelGGL(k_batch_mask_init, dim3(nblk(p.tot_words, kBlock)), dim3(kBlock), 0, s, p.mem,
+                           p.off_words, n, p.tot_words);
+    if (p.tot_rot > 0)
+        hipLaunchKernelGGL(k_batch_prepare, dim3(nblk(p.tot_rot, kBlock)), dim3(kBlock), 0, s, p.mem, p.off_rot, n,
+                           p.tot_rot);
+    if (p.tot_scells > 0)
+        hipLaunchKernelGGL((k_batch_moments_static<PR, L, STREAM>), dim3(nblk(p.tot_scells, kBlock)), dim3(kBlock),
+                           0, s, p.mem, p.off_scells, n, p.tot_scells);
+    if (p.tot_rot > 0)
+        hipLaunchKernelGGL((k_batch_moments_blades<PR, L, STREAM>), dim3(nblk(9 * p.tot_rot, kBlock)), dim3(kBlock),
+                           0, s, p.mem, p.off_rot, n, 9 * p.tot_rot);
+    constexpr int bs = 64;  // levels are small: more blocks spread over more CUs
+    for (int l = 0; l < p.n_slevels; l++) {
+        const int64_t sb = p.lv_static[l], ns = p.lv_static[l + 1] - sb, tot = p.lv_threads[l];
+        if (tot > 0)
+            hipLaunchKernelGGL(k_batch_force_level<true>, dim3(nblk(tot, bs)), dim3(bs), 0, s, p.mem, p.segs,
+                               p.seg_cum, sb, (int)ns, tot);
+    }
+    for (int l = 0; l < p.n_rlevels; l++) {
+        const int64_t sb = p.lv_blade[l], ns = p.lv_blade[l + 1] - sb, tot = p.lv_threads[p.n_slevels + l];
+        if (tot > 0)
+            hipLaunchKernelGGL(k_batch_force_level<false>, dim3(nblk(tot, bs)), dim3(bs), 0, s, p.mem, p.segs,
+                               p.seg_cum, sb, (int)ns, tot);
+    }
+    hipLaunchKernelGGL(k_batch_advance<STREAM>, dim3(nblk(n, 64)), dim3(64), 0, s, p.mem_rw, n);
+    return hipGetLastError();
+}
+
+template <typename Fn>
+hipError_t with_batch_prec(int prec, int layout, int streaming, Fn&& fn) {
+    auto by_layout = [&](auto pr) -> hipError_t {
+        using PR = decltype(pr);
+        if (layout == LayoutIJKL)
+            return streaming == StreamEsotwist ? fn(PR{}, IC<LayoutIJKL>{}, IC<StreamEsotwist>{})
+                                               : fn(PR{}, IC<LayoutIJKL>{}, IC<StreamSimple>{});
+        return streaming == StreamEsotwist ? fn(PR{}, IC<LayoutLIJK>{}, IC<StreamEsotwist>{})
+                                           : fn(PR{}, IC<LayoutLIJK>{}, IC<StreamSimple>{});
+    };
+    switch (prec) {
+        case 0: return by_layout(PrecF32{});
+        case 2: return by_layout(PrecF16{});
+        case 3: return by_layout(PrecBF16{});
+        default: return hipErrorInvalidValue;
+    }
+}
+}  // namespace
+
+int64_t batch_coll_blocks(const Grid& g, int tile) {
+    if (tile == TILE_ROW64) {
+        const int64_t segs = (g.zg1 + 63) / 64;
+        const int64_t waves = (int64_t)g.xg1 * g.yg1 * segs;
+        return (waves + kBlock / 64 - 1) / (kBlock / 64);
+    }
+    const int64_t plane = (int64_t)g.yg * g.zg;
+    return (int64_t)g.xg1 * ((plane + kBlock - 1) / kBlock);
+}
+
+hipError_t launch_batch_step(int prec, int layout, int streaming, const BatchPlan& plan, hipStream_t s) {
+    return with_batch_prec(prec, layout, streaming, [&](auto pr, auto lc, auto sc) {
+        return batch_step_t<decltype(pr), decltype(lc)::value, decltype(sc)::value>(plan, s);
+    });
+}
+
+hipError_t launch_batch_fused(int prec, int layout, int streaming, const BatchPlan& plan, hipStream_t s) {
+    return with_batch_prec(prec, layout, streaming, [&](auto pr, auto lc, auto sc) {
+        return batch_fused_t<decltype(pr), decltype(lc)::value, decltype(sc)::value>(plan, s);
+    });
 }
 
 }  // namespace tdlb
