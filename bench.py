@@ -191,6 +191,60 @@ def run_batch_workload(td, hip, n_tanks, steps, precision, streaming, layout, so
                 tiles={int(e): t for e, t in zip([s["edge"] for s in specs], info["tiles"])})
 
 
+def serve_requests(duration, rate=4.0, seed=1):
+    """Poisson arrivals at `rate` per second over `duration` seconds: grid edge
+    log-uniform in [64, 320], tip speed U[0.05, 0.12], steps U[500, 5000],
+    Re U[2e3, 2e4] (numpy PCG64 seeded with `seed`)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            return out
+        edge = int(round(float(np.exp(rng.uniform(np.log(64), np.log(320))))))
+        out.append(dict(t=t, edge=edge, tip=float(rng.uniform(0.05, 0.12)), re=float(rng.uniform(2e3, 2e4)),
+                        steps=int(rng.integers(500, 5001))))
+
+
+def run_serve_workload(td, duration, rate, tick_steps, max_resident, seed=1):
+    """Feed the arrivals to a tdlb server in real time for `duration` seconds;
+    latency = finish - submit of the requests completed inside the window."""
+    import numpy as np
+    reqs = serve_requests(duration, rate, seed)
+    s = td.Server(td.PREC_F16, td.LAYOUT_LIJK, td.STREAM_ESOTWIST, tick_steps=tick_steps, max_resident=max_resident)
+    t0 = time.time()
+    i, results, ticks, last = 0, [], 0, 0.0
+    while True:
+        now = time.time() - t0
+        while i < len(reqs) and reqs[i]["t"] <= now:
+            r = reqs[i]
+            s.submit(r["edge"], r["tip"], r["re"], r["steps"])
+            i += 1
+        if now >= duration:
+            break
+        resident, queued, n = s.tick()
+        ticks += 1
+        results += s.poll()
+        if resident == 0 and queued == 0 and i < len(reqs):
+            time.sleep(max(0.0, min(0.05, reqs[i]["t"] - (time.time() - t0))))
+        if now - last >= 30:
+            print(f"[serve] t={now:.0f}s submitted {i} done {len(results)} resident {resident} queued {queued}",
+                  file=sys.stderr, flush=True)
+            last = now
+    elapsed = time.time() - t0
+    stats = s.stats()
+    s.close()
+    ok = [r for r in results if r["error"] == 0]
+    lat = np.array([r["finish_s"] - r["submit_s"] for r in ok]) if ok else np.array([0.0])
+    updates = sum(r["edge"] ** 3 * r["steps"] for r in ok)
+    return dict(duration_s=round(elapsed, 1), rate_per_s=rate, submitted=i, completed=len(ok),
+                failed=len(results) - len(ok), backlog=i - len(results), ticks=ticks, tick_steps=tick_steps,
+                max_resident=max_resident, p50_latency_s=round(float(np.percentile(lat, 50)), 2),
+                p99_latency_s=round(float(np.percentile(lat, 99)), 2), aggregate_mlups=updates / elapsed / 1e6,
+                budget_gb=round(stats["budget_bytes"] / 1e9, 1))
+
+
 def copy_calibration(gib):
     """A plain device copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel
     give the factor between those counters and bytes moved."""
@@ -218,8 +272,13 @@ def main():
     ap.add_argument("--variants", default="all", help="all, precision (fp32 + fp16/bf16 Esotwist/LIJK) or headline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--workload", default="tank", choices=["tank", "batch"],
-                    help="tank: one 512^3 tank (headline); batch: independent tanks batched vs one at a time")
+    ap.add_argument("--workload", default="tank", choices=["tank", "batch", "serve"],
+                    help="tank: one 512^3 tank (headline); batch: independent tanks batched vs one at a time; "
+                         "serve: a Poisson stream of tank requests")
+    ap.add_argument("--duration", type=float, default=600.0, help="serve: seconds of arrivals")
+    ap.add_argument("--rate", type=float, default=4.0, help="serve: arrivals per second")
+    ap.add_argument("--tick-steps", type=int, default=50, help="serve: steps per scheduling tick")
+    ap.add_argument("--max-resident", type=int, default=0, help="serve: tanks at once (0: memory-bound; 1: one at a time)")
     ap.add_argument("--tanks", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--solo-sample", type=int, default=0,
@@ -247,6 +306,18 @@ def main():
                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f16-store/f32",
                "data": "synthetic: Rushton tanks at rest, geometry from this library's generator",
                "config": {"workload": f"batched_rushton_tanks_{args.tanks}_fp16_esotwist_lijk", **r}}
+        print(json.dumps(out), flush=True)
+        return 0
+    if args.workload == "serve":
+        r = run_serve_workload(td, args.duration, args.rate, args.tick_steps, args.max_resident)
+        print(f"[bench] serve: p50 {r['p50_latency_s']} s, p99 {r['p99_latency_s']} s, "
+              f"{r['aggregate_mlups']:.0f} MLUPS, {r['completed']}/{r['submitted']} done",
+              file=sys.stderr, flush=True)
+        out = {"metric": METRIC, "value": round(r["aggregate_mlups"], 1), "unit": "MLUPS", "n_gpus": 1,
+               "steps": r["ticks"], "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f16-store/f32",
+               "data": "synthetic: Poisson stream of Rushton tank requests, geometry from this library's generator",
+               "config": {"workload": "served_rushton_tanks_fp16_esotwist_lijk", **r}}
         print(json.dumps(out), flush=True)
         return 0
     F32, F16, BF16 = td.PREC_F32, td.PREC_F16, td.PREC_BF16

@@ -419,3 +419,97 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class TankRequest(ctypes.Structure):
+    _fields_ = [("edge", ctypes.c_int64), ("tip_speed", ctypes.c_double), ("reynolds", ctypes.c_double),
+                ("steps", ctypes.c_int64), ("startup_steps", ctypes.c_int64)]
+
+
+class TankResult(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_int64), ("edge", ctypes.c_int64), ("steps", ctypes.c_int64),
+                ("error", ctypes.c_int32)] + \
+               [(n, ctypes.c_double) for n in ("submit_s", "admit_s", "start_s", "finish_s", "mass", "mass_drift",
+                                               "torque", "power_number")]
+
+
+def _server_sigs(L):
+    vp, i64, i32, d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double
+    P = ctypes.POINTER
+    sig = {
+        "tdlb_server_create": ([i32, i32, i32, i32, d, i64, i64, i32, P(vp)], ctypes.c_int),
+        "tdlb_server_destroy": ([vp], ctypes.c_int),
+        "tdlb_server_submit": ([vp, P(TankRequest), P(i64)], ctypes.c_int),
+        "tdlb_server_tick": ([vp, P(i64), P(i64), P(i64)], ctypes.c_int),
+        "tdlb_server_poll": ([vp, P(TankResult), i64, P(i64)], ctypes.c_int),
+        "tdlb_server_take": ([vp, i64, P(vp)], ctypes.c_int),
+        "tdlb_server_stats": ([vp, P(d), P(d), P(d)], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+
+
+class TankHandle(ComputeUnit):
+    """A ComputeUnit handle the server created (tdlb_server_take); same methods."""
+
+    def __init__(self, h, edge, precision):
+        self._h = h
+        xg, yg, zg, size = (ctypes.c_int64() for _ in range(4))
+        check(lib().tdlb_cu_dims(self._h, ctypes.byref(xg), ctypes.byref(yg), ctypes.byref(zg), ctypes.byref(size)))
+        self.xg, self.yg, self.zg, self.size = xg.value, yg.value, zg.value, size.value
+        self.precision = precision
+        self.np_dtype = np.float64 if precision == PREC_F64 else np.float32
+        self.step, self.angle = 0, 0.0
+
+
+class Server:
+    """Serving loop over tdlb_server_*: submit requests, tick, poll results."""
+
+    def __init__(self, precision=PREC_F16, layout=LAYOUT_LIJK, streaming=STREAM_ESOTWIST, device_id=0,
+                 hbm_budget=0.0, tick_steps=50, max_resident=0, keep_handles=False):
+        _server_sigs(lib())
+        self._s = ctypes.c_void_p()
+        self.precision = precision
+        check(lib().tdlb_server_create(precision, layout, streaming, device_id, float(hbm_budget), tick_steps,
+                                       max_resident, 1 if keep_handles else 0, ctypes.byref(self._s)))
+
+    def submit(self, edge, tip_speed, reynolds, steps, startup_steps=100):
+        r = TankRequest(edge, tip_speed, reynolds, steps, startup_steps)
+        t = ctypes.c_int64()
+        check(lib().tdlb_server_submit(self._s, ctypes.byref(r), ctypes.byref(t)))
+        return t.value
+
+    def tick(self):
+        """(resident, queued, steps_run) after one tick."""
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib().tdlb_server_tick(self._s, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def poll(self, cap=256):
+        buf = (TankResult * cap)()
+        n = ctypes.c_int64()
+        check(lib().tdlb_server_poll(self._s, buf, cap, ctypes.byref(n)))
+        return [{f: getattr(buf[a], f) for f, _ in TankResult._fields_} for a in range(n.value)]
+
+    def take(self, ticket, edge):
+        h = ctypes.c_void_p()
+        check(lib().tdlb_server_take(self._s, ticket, ctypes.byref(h)))
+        return TankHandle(h, edge, self.precision)
+
+    def stats(self):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(lib().tdlb_server_stats(self._s, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"now_s": a.value, "reserved_bytes": b.value, "budget_bytes": c.value}
+
+    def close(self):
+        if getattr(self, "_s", None) and self._s.value:
+            check(lib().tdlb_server_destroy(self._s))
+            self._s = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -86,8 +86,23 @@ int prof_mark(tdlb_cu* cu) {
 }
 
 // ---- phase handling
+// The explicit Q of the 16-bit storage is allocated on first use.  A fresh one
+// holds what the reference's Q holds in the slots the lattice never rewrites:
+// the initial moments, and for Simple the bounce-back rims settled from them.
+int ensure_m(tdlb_cu* cu) {
+    if (cu->M) return TDLB_OK;
+    if (hipMalloc(&cu->M, cu->q_bytes()) != hipSuccess) {
+        cu->M = nullptr;
+        return fail(TDLB_ERR_NOMEM, "explicit Q of the 16-bit storage");
+    }
+    HIPCHK(launch_fill_q(cu->eprec(), cu->prm.layout, cu->M, cu->g, cu->rho0, cu->stream));
+    if (cu->simple()) HIPCHK(launch_bounce_back((int)cu->tsize, cu->prm.layout, cu->M, cu->g, cu->stream));
+    return TDLB_OK;
+}
+
 int materialise(tdlb_cu* cu) {
     if (cu->phase != PH_LAZY) return TDLB_OK;
+    if (cu->shifted()) CHK(ensure_m(cu));
     const auto c = cu->cargs();
     if (cu->simple()) {
         void* src = cu->Q[cu->cur];
@@ -414,7 +429,6 @@ int tdlb_cu_create(const tdlb_cu_params* p, const tdlb_flow* f, tdlb_cu_t* out) 
     const int nq = cu->simple() ? 2 : 1;
     for (int b = 0; b < nq; b++)
         if (hipMalloc(&cu->Q[b], cu->qs_bytes()) != hipSuccess) return bad(TDLB_ERR_NOMEM, "hipMalloc Q");
-    if (cu->shifted() && hipMalloc(&cu->M, cu->q_bytes()) != hipSuccess) return bad(TDLB_ERR_NOMEM, "hipMalloc M");
     if (hipMalloc(&cu->F, cu->f_bytes()) != hipSuccess) return bad(TDLB_ERR_NOMEM, "hipMalloc F");
     if (hipMalloc(&cu->Nu, cu->nu_bytes()) != hipSuccess) return bad(TDLB_ERR_NOMEM, "hipMalloc Nu");
     if (hipMalloc((void**)&cu->O, size) != hipSuccess) return bad(TDLB_ERR_NOMEM, "hipMalloc O");
@@ -431,7 +445,8 @@ int tdlb_cu_create(const tdlb_cu_params* p, const tdlb_flow* f, tdlb_cu_t* out) 
         return bad(TDLB_ERR_HIP, "initial fill failed");
     for (int b = 0; b < nq; b++)
         if (launch_zero(cu->Q[b], cu->qs_bytes(), cu->stream) != hipSuccess) return bad(TDLB_ERR_HIP, "Q fill failed");
-    if (cu->M && launch_zero(cu->M, cu->q_bytes(), cu->stream) != hipSuccess) return bad(TDLB_ERR_HIP, "M fill failed");
+    // the 16-bit lattice starts as the rest state at rho0 (all deviations zero)
+    if (cu->shifted()) cu->phase = PH_LAZY;
     if (hipStreamSynchronize(cu->stream) != hipSuccess) return bad(TDLB_ERR_HIP, "synchronise failed");
     *out = cu;
     return TDLB_OK;
@@ -460,8 +475,15 @@ int tdlb_cu_initialise(tdlb_cu_t cu, double rho) {
     CHK(set_device(cu));
     const double r = to_prec(cu, rho);
     if (cu->shifted()) {
+        // rest state at rho0 = rho: every stored deviation is zero
         cu->rho0 = (float)r;
-        HIPCHK(launch_fill_q(cu->eprec(), cu->prm.layout, cu->M, cu->g, r, cu->stream));
+        const int nq = cu->simple() ? 2 : 1;
+        for (int b = 0; b < nq; b++) HIPCHK(launch_zero(cu->Q[b], cu->qs_bytes(), cu->stream));
+        if (cu->M) {
+            HIPCHK(hipStreamSynchronize(cu->stream));
+            (void)hipFree(cu->M);
+            cu->M = nullptr;
+        }
     } else {
         const int nq = cu->simple() ? 2 : 1;
         for (int b = 0; b < nq; b++)
@@ -473,7 +495,9 @@ int tdlb_cu_initialise(tdlb_cu_t cu, double rho) {
     HIPCHK(launch_zero(cu->fmask[0], cu->mask_bytes(), cu->stream));
     HIPCHK(launch_zero(cu->fmask[1], cu->mask_bytes(), cu->stream));
     HIPCHK(launch_zero(cu->gmask, cu->mask_bytes(), cu->stream));
-    cu->phase = PH_EXPLICIT;
+    // the 16-bit storage starts from its populations: the rest populations have
+    // the moments initialise writes (rho0, 0, ..., 0)
+    cu->phase = cu->shifted() ? PH_LAZY : PH_EXPLICIT;
     cu->o_dirty = false;
     cu->static_in_prev = false;
     if (cu->bound) HIPCHK(launch_copy(cu->static_base, cu->static_bits, cu->mask_bytes(), cu->stream));
@@ -781,7 +805,8 @@ int tdlb_cu_bind_rushton(tdlb_cu_t cu, const tdlb_rushton_params* p, uint64_t st
                          double beta) {
     if (!cu || !p) return fail(TDLB_ERR_INVALID, "null argument");
     CHK(set_device(cu));
-    CHK(materialise(cu));
+    // the 16-bit storage reads density and momentum from its populations instead
+    if (!cu->shifted()) CHK(materialise(cu));
     Rushton R(*p);
     const BladeEnum be = R.blade_enum();
     if (p->n_blades < 1 || p->n_blades > 16) return fail(TDLB_ERR_INVALID, "n_blades must be 1..16");
@@ -873,9 +898,9 @@ int tdlb_cu_bind_rushton(tdlb_cu_t cu, const tdlb_rushton_params* p, uint64_t st
     cu->alfa = alfa;
     cu->beta = beta;
 
-    // initial forcing of main.cpp:305 on the (explicit) moments
+    // initial forcing of main.cpp:305 on the current density and momentum
     CHK(gen_blades(cu, angle, delta));
-    CHK(bound_forcing(cu, MOMSRC_STORED, true));
+    CHK(bound_forcing(cu, cell_source(cu).momsrc, true));
     return TDLB_OK;
 }
 
