@@ -83,6 +83,7 @@ struct tdlb_batch {
     BatchPlan plan{};
     std::vector<int64_t> lv_static, lv_blade, lv_threads;
     std::vector<std::pair<tdlb_cu*, int32_t>> key;  // members (with budget) and their tiles
+    std::vector<int64_t> launch_sig;                 // launch sizes and arguments of the captured step
     bool buffers_moved = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -291,10 +292,10 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     // forcing levels: level l of every member that has it
     std::vector<LevelSeg> segs;
     std::vector<int64_t> cum;
-    b->lv_static.assign(1, 0);
-    b->lv_blade.assign(1, 0);
     b->lv_threads.clear();
+    // lv[l] .. lv[l + 1] are the segments of level l; the blade levels follow the static ones in segs
     auto add_levels = [&](bool stat, std::vector<int64_t>& lv) {
+        lv.assign(1, (int64_t)segs.size());
         size_t nl = 0;
         for (int r = 0; r < M; r++) {
             const tdlb_cu* cu = b->members[idx[r]].cu;
@@ -319,7 +320,9 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     };
     add_levels(true, b->lv_static);
     add_levels(false, b->lv_blade);
-    const size_t ntab = tab.size() + cum.size();
+    const int64_t tot_coll = off_coll[M], tot_rot = off_rot[M], tot_words = off_words[M], tot_scells = off_scells[M];
+    tab.insert(tab.end(), cum.begin(), cum.end());  // may move tab: the off_* pointers are not used below
+    const size_t ntab = tab.size();
 
     int moved = 0;
     int g;
@@ -331,7 +334,6 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     moved |= g;
     if (b->d_steps.grow(std::max<size_t>(steps.size(), 1)) < 0) return fail(TDLB_ERR_NOMEM, "batch blade table");
     for (int r = 0; r < M; r++) mem[r].steps = b->d_steps.p + step_base[r];
-    tab.insert(tab.end(), cum.begin(), cum.end());
     if (M > 0) HIPCHK(hipMemcpyAsync(b->d_mem.p, mem.data(), M * sizeof(MemberDev), hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipMemcpyAsync(b->d_tab.p, tab.data(), tab.size() * sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
     if (!segs.empty())
@@ -352,10 +354,10 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     p.off_scells = b->d_tab.p + 3 * (M + 1);
     p.seg_cum = b->d_tab.p + 4 * (M + 1);
     p.segs = b->d_segs.p;
-    p.tot_coll = off_coll[M];
-    p.tot_rot = off_rot[M];
-    p.tot_words = off_words[M];
-    p.tot_scells = off_scells[M];
+    p.tot_coll = tot_coll;
+    p.tot_rot = tot_rot;
+    p.tot_words = tot_words;
+    p.tot_scells = tot_scells;
     p.n_slevels = (int32_t)b->lv_static.size() - 1;
     p.n_rlevels = (int32_t)b->lv_blade.size() - 1;
     p.lv_static = b->lv_static.data();
@@ -363,9 +365,19 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     p.lv_threads = b->lv_threads.data();
     // launches: blades, fused, mask, prepare, static moments, blade moments, levels, advance
     b->launches_per_step = 1 + 1 + 1 + 1 + 1 + 1 + p.n_slevels + p.n_rlevels + 1;
-    *structure_changed = moved || key != b->key || b->buffers_moved;
+    // A captured step bakes in every launch size and kernel argument: recapture
+    // whenever any of them differs (member identity alone is not enough, a freed
+    // handle's address can come back for a tank of another size).
+    std::vector<int64_t> sig = {M, tot_coll, tot_rot, tot_words, tot_scells, p.n_slevels, p.n_rlevels,
+                                (int64_t)(uintptr_t)b->d_mem.p, (int64_t)(uintptr_t)b->d_tab.p,
+                                (int64_t)(uintptr_t)b->d_segs.p};
+    sig.insert(sig.end(), b->lv_static.begin(), b->lv_static.end());
+    sig.insert(sig.end(), b->lv_blade.begin(), b->lv_blade.end());
+    sig.insert(sig.end(), b->lv_threads.begin(), b->lv_threads.end());
+    *structure_changed = moved || sig != b->launch_sig || key != b->key || b->buffers_moved;
     b->buffers_moved = false;
     b->key = key;
+    b->launch_sig = sig;
     b->active_idx = idx;
     return TDLB_OK;
 }

@@ -1356,7 +1356,7 @@ __device__ __forceinline__ bool batch_cell(const Grid& g, int tile, uint32_t b, 
     const uint32_t bpp = (plane + kBlock - 1) / kBlock;
     const uint32_t ip = b / bpp;
     const uint32_t local = (b - ip * bpp) * kBlock + threadIdx.x;
-    if (local >= plane) return false;
+    if (ip >= (uint32_t)g.xg1 || local >= plane) return false;
     i = (int)ip + 1;
     j = (int)(local / (uint32_t)g.zg);
     k = (int)(local - (uint32_t)j * (uint32_t)g.zg);
@@ -1368,6 +1368,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_fused(const MemberDev* __restr
                                                         const int64_t* __restrict__ off, int n) {
     static_assert(std::is_same<typename PR::T, float>::value, "batches compute in float");
     using S = typename PR::S;
+    if ((int64_t)blockIdx.x >= off[n]) return;
     const int m = find_slot(off, n, blockIdx.x);
     const MemberDev& md = mem[m];
     if (!md.active) return;
@@ -1387,7 +1388,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_fused(const MemberDev* __restr
 __global__ void k_batch_blades(const MemberDev* __restrict__ mem, const int64_t* __restrict__ off, int n,
                                int64_t total) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= total) return;
+    if (t >= total || t >= off[n]) return;
     const int m = find_slot(off, n, t);
     const MemberDev& md = mem[m];
     if (!md.active) return;
@@ -1398,7 +1399,7 @@ __global__ void k_batch_blades(const MemberDev* __restrict__ mem, const int64_t*
 __global__ void k_batch_mask_init(const MemberDev* __restrict__ mem, const int64_t* __restrict__ off, int n,
                                   int64_t total) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= total) return;
+    if (t >= total || t >= off[n]) return;
     const int m = find_slot(off, n, t);
     const MemberDev& md = mem[m];
     if (!md.active) return;
@@ -1408,7 +1409,7 @@ __global__ void k_batch_mask_init(const MemberDev* __restrict__ mem, const int64
 __global__ void k_batch_prepare(const MemberDev* __restrict__ mem, const int64_t* __restrict__ off, int n,
                                 int64_t total) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= total) return;
+    if (t >= total || t >= off[n]) return;
     const int m = find_slot(off, n, t);
     const MemberDev& md = mem[m];
     if (!md.active) return;
@@ -1434,7 +1435,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_moments_static(const MemberDev
                                                                  const int64_t* __restrict__ off, int n,
                                                                  int64_t total) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= total) return;
+    if (t >= total || t >= off[n]) return;
     const int m = find_slot(off, n, t);
     const MemberDev& md = mem[m];
     if (!md.active) return;
@@ -1455,7 +1456,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_moments_blades(const MemberDev
                                                                  const int64_t* __restrict__ off, int n,
                                                                  int64_t total9) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= total9) return;
+    if (t >= total9 || t >= 9 * off[n]) return;
     const int m = find_slot(off, n, t / 9);
     const MemberDev& md = mem[m];
     if (!md.active) return;
@@ -1478,6 +1479,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_force_level(const MemberDev* _
     if (t >= total) return;
     const int64_t s = sb + find_slot(cum + sb, ns, t);
     const LevelSeg sg = segs[s];
+    if (t - cum[s] >= sg.count) return;
     const MemberDev& md = mem[sg.member];
     if (!md.active) return;
     const int64_t q = sg.begin + (t - cum[s]);
