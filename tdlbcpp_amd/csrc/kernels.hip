@@ -31,6 +31,17 @@ inline uint32_t blocks_for(uint64_t n, int bs = kBlock) {
     return (uint32_t)(b > 0x7fffffffull ? 0x7fffffffull : (b ? b : 1));
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b to XCD b % 8),
+// each XCD with its own L2.  Renumber so that each XCD takes one contiguous
+// range of rows: neighbouring k-rows share the 128 B lines that straddle them
+// (a row of 514 cells is not line aligned), and meet in the same L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
+    const uint32_t x = b % kXcds, q = b / kXcds;
+    const uint32_t per = n / kXcds, rem = n % kXcds;
+    return x * per + min(x, rem) + q;
+}
+
 // ------------------------------------------------------------------ precisions
 // S: stored population type, T: arithmetic and F/Nu type, D: type the
 // reference's double literals promote to.  kShift: populations are stored as
@@ -178,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_esotwist(typename PR::S* __restrict_
                                                      typename PR::T* __restrict__ Nu,
                                                      const uint64_t* __restrict__ fmask, Grid g,
                                                      CollideParams<typename PR::T> P, uint32_t nkb, float rho0) {
-    const RowBlock rb = row_block(g, blockIdx.x, nkb, blockDim.x);
+    const RowBlock rb = row_block(g, xcd_block(blockIdx.x, gridDim.x), nkb, blockDim.x);
     const int k = rb.k0 + (int)threadIdx.x;
     if (k > g.zg1) return;
     esotwist_cell<PR, L, MODE, EVEN>(Q, mout, F, Nu, fmask, g, P, rho0, rb.i, rb.j, k);
@@ -255,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_simple(const typename PR::S* src, ty
                                                    typename PR::T* __restrict__ Nu,
                                                    const uint64_t* __restrict__ fmask, Grid g,
                                                    CollideParams<typename PR::T> P, uint32_t nkb, float rho0) {
-    const RowBlock rb = row_block(g, blockIdx.x, nkb, blockDim.x);
+    const RowBlock rb = row_block(g, xcd_block(blockIdx.x, gridDim.x), nkb, blockDim.x);
     const int k = rb.k0 + (int)threadIdx.x;
     if (k > g.zg1) return;
     simple_cell<PR, L, MODE>(src, dst, mout, F, Nu, fmask, g, P, rho0, rb.i, rb.j, k);
@@ -693,55 +704,79 @@ __global__ __launch_bounds__(kBlock) void k_ib_moments_points(PointsDev p, int64
     cache[t] = m;
 }
 
-// Same arithmetic as k_force_level; stencil cell s = (k1+1)*3 + (i1+1) reads
-// cache[sidx[9q+s]] (INDEXED) or cache[9(q-cache_base)+s].
+// Same arithmetic as k_force_level, one point per 9 lanes of a wavefront (lane
+// s of the group holds stencil cell s = (k1+1)*3 + (i1+1)), so the gathers of a
+// point run in parallel.  Stencil cell s reads cache[sidx[9q+s]] (INDEXED) or
+// cache[9(q-cache_base)+s].  Every lane of the group adds the 9 terms in the
+// reference's order (s = 0..8, each product rounded before the add), so the
+// sums are those of the sequential loop bit for bit.  The 9 cells of a point
+// are distinct (the cached path needs 3 interior cells in x and z) and the
+// points of a level have disjoint stencils, so each F entry has one writer.
+constexpr int kGroup = 9;                  // lanes per point
+constexpr int kGroupsPerWave = 64 / kGroup;  // 7 points per wavefront, lane 63 idle
+
 template <typename T, typename D, bool INDEXED>
-__device__ __forceinline__ void force_point_cached(const PointsDev& p, int64_t q, const int32_t* __restrict__ sidx,
-                                                   int64_t cache_base, const Mom4<T>* __restrict__ cache,
-                                                   T* __restrict__ F, const Grid& g, T alfa, T beta) {
-    const int pi = p.i[q], pj = p.j[q], pk = p.k[q];
-    const T ifr = static_cast<const T*>(p.ifrac)[q], kfr = static_cast<const T*>(p.kfrac)[q];
-    T wx[3], wz[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        wx[a] = ib_weight<T>(-ifr + T(a - 1));
-        wz[a] = ib_weight<T>(-kfr + T(a - 1));
-    }
-    uint64_t cell[9];
-    Mom4<T> mom[9];
-    T fo[9][3];
-#pragma unroll
-    for (int s = 0; s < 9; s++) {
-        const int i2 = wrap_ik(pi + (s % 3) - 1, g.xg1), k2 = wrap_ik(pk + (s / 3) - 1, g.zg1);
-        cell[s] = cell_index(g, i2, pj, k2);
+__device__ __forceinline__ void force_point9(const PointsDev& p, int64_t q, bool valid, int s, int base,
+                                             const int32_t* __restrict__ sidx, int64_t cache_base,
+                                             const Mom4<T>* __restrict__ cache, T* __restrict__ F, const Grid& g,
+                                             T alfa, T beta) {
+    T w = T(0), tr = T(0), tx = T(0), ty = T(0), tz = T(0), fo0 = T(0), fo1 = T(0), fo2 = T(0);
+    T u = T(0), v = T(0), ww = T(0);
+    uint64_t cell = 0;
+    if (valid) {
+        const int pi = p.i[q], pj = p.j[q], pk = p.k[q];
+        const T ifr = static_cast<const T*>(p.ifrac)[q], kfr = static_cast<const T*>(p.kfrac)[q];
+        const int a = s % 3, b = s / 3;
+        w = ib_weight<T>(-ifr + T(a - 1)) * ib_weight<T>(-kfr + T(b - 1));
+        const int i2 = wrap_ik(pi + a - 1, g.xg1), k2 = wrap_ik(pk + b - 1, g.zg1);
+        cell = cell_index(g, i2, pj, k2);
         const int64_t e = INDEXED ? (int64_t)sidx[9 * q + s] : 9 * (q - cache_base) + s;
-        mom[s] = cache[e];
-        fo[s][0] = F[3 * cell[s] + 0];
-        fo[s][1] = F[3 * cell[s] + 1];
-        fo[s][2] = F[3 * cell[s] + 2];
+        const Mom4<T> m = cache[e];
+        fo0 = F[3 * cell + 0];
+        fo1 = F[3 * cell + 1];
+        fo2 = F[3 * cell + 2];
+        const T xx = (T)((D)m.x + D(0.5) * (D)fo0);
+        const T yy = (T)((D)m.y + D(0.5) * (D)fo1);
+        const T zz = (T)((D)m.z + D(0.5) * (D)fo2);
+        tr = w * m.r;
+        tx = w * xx;
+        ty = w * yy;
+        tz = w * zz;
+        u = static_cast<const T*>(p.u)[q];
+        v = static_cast<const T*>(p.v)[q];
+        ww = static_cast<const T*>(p.w)[q];
     }
     T rhoSum = T(0.0), xSum = T(0.0), ySum = T(0.0), zSum = T(0.0);
 #pragma unroll
-    for (int s = 0; s < 9; s++) {
-        const T ppp = wx[s % 3] * wz[s / 3];
-        const T xx = (T)((D)mom[s].x + D(0.5) * (D)fo[s][0]);
-        const T yy = (T)((D)mom[s].y + D(0.5) * (D)fo[s][1]);
-        const T zz = (T)((D)mom[s].z + D(0.5) * (D)fo[s][2]);
-        rhoSum += ppp * mom[s].r;
-        xSum += ppp * xx;
-        ySum += ppp * yy;
-        zSum += ppp * zz;
+    for (int t = 0; t < kGroup; t++) {
+        rhoSum += __shfl(tr, base + t, 64);
+        xSum += __shfl(tx, base + t, 64);
+        ySum += __shfl(ty, base + t, 64);
+        zSum += __shfl(tz, base + t, 64);
     }
-    xSum -= rhoSum * static_cast<const T*>(p.u)[q];
-    ySum -= rhoSum * static_cast<const T*>(p.v)[q];
-    zSum -= rhoSum * static_cast<const T*>(p.w)[q];
-#pragma unroll
-    for (int s = 0; s < 9; s++) {
-        const T bp = beta * (wx[s % 3] * wz[s / 3]);
-        F[3 * cell[s] + 0] = alfa * fo[s][0] - bp * xSum;
-        F[3 * cell[s] + 1] = alfa * fo[s][1] - bp * ySum;
-        F[3 * cell[s] + 2] = alfa * fo[s][2] - bp * zSum;
-    }
+    if (!valid) return;
+    xSum -= rhoSum * u;
+    ySum -= rhoSum * v;
+    zSum -= rhoSum * ww;
+    const T bp = beta * w;
+    F[3 * cell + 0] = alfa * fo0 - bp * xSum;
+    F[3 * cell + 1] = alfa * fo1 - bp * ySum;
+    F[3 * cell + 2] = alfa * fo2 - bp * zSum;
+}
+
+// lane -> (point index within the launch, stencil cell, group base lane)
+__device__ __forceinline__ int64_t group_point(int& s, int& base) {
+    const int lane = (int)(threadIdx.x & 63), grp = lane / kGroup;
+    s = lane - grp * kGroup;
+    base = grp * kGroup;
+    const int64_t wave = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+    return grp < kGroupsPerWave ? wave * kGroupsPerWave + grp : -1;
+}
+
+inline uint32_t group_blocks(int64_t points, int bs) {
+    const int64_t waves = (points + kGroupsPerWave - 1) / kGroupsPerWave;
+    const int64_t per_block = bs / 64;
+    return (uint32_t)((waves + per_block - 1) / per_block);
 }
 
 template <typename T, typename D, bool INDEXED>
@@ -749,9 +784,11 @@ __global__ __launch_bounds__(kBlock) void k_force_level_cached(PointsDev p, int6
                                                                const int32_t* __restrict__ sidx, int64_t cache_base,
                                                                const Mom4<T>* __restrict__ cache, T* __restrict__ F,
                                                                Grid g, T alfa, T beta) {
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    force_point_cached<T, D, INDEXED>(p, begin + t, sidx, cache_base, cache, F, g, alfa, beta);
+    int s, base;
+    const int64_t t = group_point(s, base);
+    const bool valid = t >= 0 && t < count;
+    force_point9<T, D, INDEXED>(p, begin + (valid ? t : 0), valid, s, base, sidx, cache_base, cache, F, g, alfa,
+                                beta);
 }
 
 // ------------------------------------------------------------------ FP64 sums and macroscopic fields
@@ -1221,8 +1258,8 @@ hipError_t launch_force_level_cached(int prec, const PointsDev& p, int64_t begin
                                      const int32_t* sidx, int64_t cache_base, const void* cache, void* F,
                                      const Grid& g, double alfa, double beta, hipStream_t s) {
     if (count <= 0) return hipSuccess;
-    constexpr int bs = 64;  // levels are small: more blocks spread over more CUs
-    const uint32_t nb = (uint32_t)((count + bs - 1) / bs);
+    constexpr int bs = 64;  // one wavefront per block: levels are small, spread them over the CUs
+    const uint32_t nb = group_blocks(count, bs);
     TDLB_WITH_PREC(prec, {
         using T = typename PR::T;
         using D = typename PR::D;
@@ -1368,12 +1405,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_fused(const MemberDev* __restr
                                                         const int64_t* __restrict__ off, int n) {
     static_assert(std::is_same<typename PR::T, float>::value, "batches compute in float");
     using S = typename PR::S;
-    if ((int64_t)blockIdx.x >= off[n]) return;
-    const int m = find_slot(off, n, blockIdx.x);
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    if ((int64_t)blk >= off[n]) return;
+    const int m = find_slot(off, n, blk);
     const MemberDev& md = mem[m];
     if (!md.active) return;
     int i, j, k;
-    if (!batch_cell(md.g, md.tile, (uint32_t)(blockIdx.x - off[m]), i, j, k)) return;
+    if (!batch_cell(md.g, md.tile, (uint32_t)(blk - off[m]), i, j, k)) return;
     const uint64_t* fm = md.fmask[md.fm];
     if constexpr (STREAM == StreamEsotwist) {
         S* Q = static_cast<S*>(md.Q[0]);
@@ -1475,21 +1513,23 @@ __global__ __launch_bounds__(kBlock) void k_batch_force_level(const MemberDev* _
                                                               const LevelSeg* __restrict__ segs,
                                                               const int64_t* __restrict__ cum, int64_t sb, int ns,
                                                               int64_t total) {
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    const int64_t s = sb + find_slot(cum + sb, ns, t);
+    int sl, base;
+    const int64_t t = group_point(sl, base);
+    bool valid = t >= 0 && t < total;
+    // every lane of the wavefront reaches the group shuffles; lanes without a point pass valid = false
+    const int64_t s = valid ? sb + find_slot(cum + sb, ns, t) : sb;
     const LevelSeg sg = segs[s];
-    if (t - cum[s] >= sg.count) return;
+    valid = valid && t - cum[s] < sg.count;
     const MemberDev& md = mem[sg.member];
-    if (!md.active) return;
-    const int64_t q = sg.begin + (t - cum[s]);
+    valid = valid && md.active;
+    const int64_t q = valid ? sg.begin + (t - cum[s]) : 0;
     if (INDEXED)
-        force_point_cached<float, double, true>(md.pts, q, md.sidx, 0, static_cast<const Mom4<float>*>(md.scache),
-                                                md.F, md.g, md.alfa, md.beta);
+        force_point9<float, double, true>(md.pts, q, valid, sl, base, md.sidx, 0,
+                                          static_cast<const Mom4<float>*>(md.scache), md.F, md.g, md.alfa, md.beta);
     else
-        force_point_cached<float, double, false>(md.pts, q, nullptr, md.n_static,
-                                                 static_cast<const Mom4<float>*>(md.rcache), md.F, md.g, md.alfa,
-                                                 md.beta);
+        force_point9<float, double, false>(md.pts, q, valid, sl, base, nullptr, md.n_static,
+                                           static_cast<const Mom4<float>*>(md.rcache), md.F, md.g, md.alfa,
+                                           md.beta);
 }
 
 template <int STREAM>
@@ -1539,14 +1579,14 @@ hipError_t batch_step_t(const BatchPlan& p, hipStream_t s) {
     for (int l = 0; l < p.n_slevels; l++) {
         const int64_t sb = p.lv_static[l], ns = p.lv_static[l + 1] - sb, tot = p.lv_threads[l];
         if (tot > 0)
-            hipLaunchKernelGGL(k_batch_force_level<true>, dim3(nblk(tot, bs)), dim3(bs), 0, s, p.mem, p.segs,
-                               p.seg_cum, sb, (int)ns, tot);
+            hipLaunchKernelGGL(k_batch_force_level<true>, dim3(group_blocks(tot, bs)), dim3(bs), 0, s, p.mem,
+                               p.segs, p.seg_cum, sb, (int)ns, tot);
     }
     for (int l = 0; l < p.n_rlevels; l++) {
         const int64_t sb = p.lv_blade[l], ns = p.lv_blade[l + 1] - sb, tot = p.lv_threads[p.n_slevels + l];
         if (tot > 0)
-            hipLaunchKernelGGL(k_batch_force_level<false>, dim3(nblk(tot, bs)), dim3(bs), 0, s, p.mem, p.segs,
-                               p.seg_cum, sb, (int)ns, tot);
+            hipLaunchKernelGGL(k_batch_force_level<false>, dim3(group_blocks(tot, bs)), dim3(bs), 0, s, p.mem,
+                               p.segs, p.seg_cum, sb, (int)ns, tot);
     }
     hipLaunchKernelGGL(k_batch_advance<STREAM>, dim3(nblk(n, 64)), dim3(64), 0, s, p.mem_rw, n);
     return hipGetLastError();
