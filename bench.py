@@ -329,6 +329,8 @@ def main():
         variants += [("Esotwist/IJKL", td.STREAM_ESOTWIST, td.LAYOUT_IJKL, F32),
                      ("Simple/LIJK", td.STREAM_SIMPLE, td.LAYOUT_LIJK, F32),
                      ("Simple/IJKL", td.STREAM_SIMPLE, td.LAYOUT_IJKL, F32),
+                     ("Esotwist/IJKL_SOA", td.STREAM_ESOTWIST, td.LAYOUT_IJKL_SOA, F32),
+                     ("Simple/IJKL_SOA", td.STREAM_SIMPLE, td.LAYOUT_IJKL_SOA, F32),
                      ("Simple/LIJK/fp16", td.STREAM_SIMPLE, td.LAYOUT_LIJK, F16)]
     results = {}
     for name, st, ly, prec in variants:

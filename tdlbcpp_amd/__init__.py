@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtdlb_hip.so")
 
-LAYOUT_IJKL, LAYOUT_LIJK = 0, 1
+LAYOUT_IJKL, LAYOUT_LIJK, LAYOUT_IJKL_SOA = 0, 1, 2  # IJKL_SOA: IJKL at the boundary, LIJK in HBM
 STREAM_SIMPLE, STREAM_ESOTWIST = 0, 1
 PREC_F32, PREC_F64, PREC_F16, PREC_BF16 = 0, 1, 2, 3
 FIELD_Q, FIELD_F, FIELD_NU, FIELD_O = 0, 1, 2, 3
@@ -304,13 +304,13 @@ class ComputeUnit:
 
     def q_ijkl(self, dtype=None):
         q = self.get(FIELD_Q, dtype)
-        if self.layout == LAYOUT_IJKL:
+        if self.layout in (LAYOUT_IJKL, LAYOUT_IJKL_SOA):
             return q.reshape(self.xg, self.yg, self.zg, 18)
         return np.moveaxis(q.reshape(18, self.xg, self.yg, self.zg), 0, -1)
 
     def set_q_ijkl(self, q4):
         q4 = np.asarray(q4)
-        if self.layout == LAYOUT_IJKL:
+        if self.layout in (LAYOUT_IJKL, LAYOUT_IJKL_SOA):
             self.set(FIELD_Q, np.ascontiguousarray(q4).reshape(-1))
         else:
             self.set(FIELD_Q, np.ascontiguousarray(np.moveaxis(q4, -1, 0)).reshape(-1))

@@ -71,6 +71,7 @@ struct tdlb_batch {
     int prec = 0, layout = 0, streaming = 0, device = 0;
     hipStream_t stream = nullptr;
     std::vector<Member> members;
+    int dlayout() const { return layout == TDLB_LAYOUT_IJKL ? LayoutIJKL : LayoutLIJK; }  // layout in HBM
     bool use_graph = true;
     bool autotune = true;
 
@@ -206,11 +207,11 @@ int tune_tile(tdlb_batch* b, const tdlb_cu* cu, int32_t* out) {
         ok = hipMemcpyAsync(dmem, &md, sizeof(md), hipMemcpyHostToDevice, b->stream) == hipSuccess &&
              hipMemcpyAsync(dtab, tab, sizeof(tab), hipMemcpyHostToDevice, b->stream) == hipSuccess;
         for (int w = 0; w < 2 && ok; w++)
-            ok = launch_batch_fused(b->prec, b->layout, b->streaming, p, b->stream) == hipSuccess;
+            ok = launch_batch_fused(b->prec, b->dlayout(), b->streaming, p, b->stream) == hipSuccess;
         ok = ok && hipEventRecord(e0, b->stream) == hipSuccess;
         constexpr int reps = 5;
         for (int w = 0; w < reps && ok; w++)
-            ok = launch_batch_fused(b->prec, b->layout, b->streaming, p, b->stream) == hipSuccess;
+            ok = launch_batch_fused(b->prec, b->dlayout(), b->streaming, p, b->stream) == hipSuccess;
         ok = ok && hipEventRecord(e1, b->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
         float ms = 0.f;
         ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
@@ -383,7 +384,7 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
 }
 
 int record_step(tdlb_batch* b) {
-    HIPCHK(launch_batch_step(b->prec, b->layout, b->streaming, b->plan, b->stream));
+    HIPCHK(launch_batch_step(b->prec, b->dlayout(), b->streaming, b->plan, b->stream));
     return TDLB_OK;
 }
 
@@ -417,7 +418,8 @@ int tdlb_batch_create(int32_t precision, int32_t layout, int32_t streaming, int3
     if (!out) return fail(TDLB_ERR_INVALID, "null argument");
     if (precision != TDLB_PREC_F32 && precision != TDLB_PREC_F16 && precision != TDLB_PREC_BF16)
         return fail(TDLB_ERR_UNSUPPORTED, "batches run the float-arithmetic precisions (fp32, fp16, bf16)");
-    if (layout != TDLB_LAYOUT_IJKL && layout != TDLB_LAYOUT_LIJK) return fail(TDLB_ERR_INVALID, "layout");
+    if (layout != TDLB_LAYOUT_IJKL && layout != TDLB_LAYOUT_LIJK && layout != TDLB_LAYOUT_IJKL_SOA)
+        return fail(TDLB_ERR_INVALID, "layout");
     if (streaming != TDLB_STREAM_SIMPLE && streaming != TDLB_STREAM_ESOTWIST) return fail(TDLB_ERR_INVALID, "streaming");
     tdlb_batch* b = new tdlb_batch();
     b->prec = precision;

@@ -95,8 +95,8 @@ int ensure_m(tdlb_cu* cu) {
         cu->M = nullptr;
         return fail(TDLB_ERR_NOMEM, "explicit Q of the 16-bit storage");
     }
-    HIPCHK(launch_fill_q(cu->eprec(), cu->prm.layout, cu->M, cu->g, cu->rho0, cu->stream));
-    if (cu->simple()) HIPCHK(launch_bounce_back((int)cu->tsize, cu->prm.layout, cu->M, cu->g, cu->stream));
+    HIPCHK(launch_fill_q(cu->eprec(), cu->dlay(), cu->M, cu->g, cu->rho0, cu->stream));
+    if (cu->simple()) HIPCHK(launch_bounce_back((int)cu->tsize, cu->dlay(), cu->M, cu->g, cu->stream));
     return TDLB_OK;
 }
 
@@ -107,16 +107,16 @@ int materialise(tdlb_cu* cu) {
     if (cu->simple()) {
         void* src = cu->Q[cu->cur];
         void* dst = cu->shifted() ? cu->M : cu->Q[1 - cu->cur];
-        HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_MOM_PULL, src, dst, dst, cu->F, cu->Nu,
+        HIPCHK(launch_simple(cu->prec(), cu->dlay(), MODE_MOM_PULL, src, dst, dst, cu->F, cu->Nu,
                              cu->fmask[cu->fm], cu->g, c, cu->rho0, cu->stream));
         if (cu->shifted()) {
-            HIPCHK(launch_ghost_fill_decode(cu->prec(), cu->prm.layout, src, dst, cu->g, cu->rho0, cu->stream));
+            HIPCHK(launch_ghost_fill_decode(cu->prec(), cu->dlay(), src, dst, cu->g, cu->rho0, cu->stream));
         } else {
-            HIPCHK(launch_ghost_fill((int)cu->ssize, cu->prm.layout, src, dst, cu->g, cu->stream));
+            HIPCHK(launch_ghost_fill((int)cu->ssize, cu->dlay(), src, dst, cu->g, cu->stream));
             cu->cur = 1 - cu->cur;
         }
     } else {
-        HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_MOM, cu->even ? 1 : 0, cu->Q[0], cu->eq(), cu->F,
+        HIPCHK(launch_esotwist(cu->prec(), cu->dlay(), MODE_MOM, cu->even ? 1 : 0, cu->Q[0], cu->eq(), cu->F,
                                cu->Nu, cu->fmask[cu->fm], cu->g, c, cu->rho0, cu->stream));
     }
     cu->phase = PH_EXPLICIT;
@@ -296,10 +296,10 @@ int forcing_pass(tdlb_cu* cu, const DevPoints& dp, const std::vector<ForceSegmen
     if (!tiny) {
         for (auto& sg : segs) {
             if (sg.set)
-                HIPCHK(launch_ib_moments_cells(qp, cu->prm.layout, cu->prm.streaming, momsrc, even, sg.set->cells,
+                HIPCHK(launch_ib_moments_cells(qp, cu->dlay(), cu->prm.streaming, momsrc, even, sg.set->cells,
                                                sg.set->n_cells, sg.set->cache, Qsrc, cu->g, cu->rho0, cu->stream));
             else
-                HIPCHK(launch_ib_moments_points(qp, cu->prm.layout, cu->prm.streaming, momsrc, even, dp.d,
+                HIPCHK(launch_ib_moments_points(qp, cu->dlay(), cu->prm.streaming, momsrc, even, dp.d,
                                                 sg.cache_base, sg.n_points, sg.direct_cache, Qsrc, cu->g, cu->rho0,
                                                 cu->stream));
         }
@@ -307,7 +307,7 @@ int forcing_pass(tdlb_cu* cu, const DevPoints& dp, const std::vector<ForceSegmen
     for (auto& sg : segs)
         for (auto& r : sg.levels) {
             if (tiny)
-                HIPCHK(launch_force_level(qp, cu->prm.layout, cu->prm.streaming, momsrc, even, dp.d, r.first,
+                HIPCHK(launch_force_level(qp, cu->dlay(), cu->prm.streaming, momsrc, even, dp.d, r.first,
                                           r.second - r.first, cu->F, Qsrc, cu->g, cu->alfa, cu->beta, cu->rho0,
                                           cu->stream));
             else
@@ -391,7 +391,8 @@ int tdlb_cu_create(const tdlb_cu_params* p, const tdlb_flow* f, tdlb_cu_t* out) 
     if (!p || !f || !out) return fail(TDLB_ERR_INVALID, "null argument");
     if (p->ghost != 1) return fail(TDLB_ERR_UNSUPPORTED, "ghost must be 1 (GridParams::multiStep default)");
     if (p->x < 1 || p->y < 1 || p->z < 1) return fail(TDLB_ERR_INVALID, "grid must have at least one interior cell");
-    if (p->layout != TDLB_LAYOUT_IJKL && p->layout != TDLB_LAYOUT_LIJK) return fail(TDLB_ERR_INVALID, "layout");
+    if (p->layout != TDLB_LAYOUT_IJKL && p->layout != TDLB_LAYOUT_LIJK && p->layout != TDLB_LAYOUT_IJKL_SOA)
+        return fail(TDLB_ERR_INVALID, "layout");
     if (p->streaming != TDLB_STREAM_SIMPLE && p->streaming != TDLB_STREAM_ESOTWIST)
         return fail(TDLB_ERR_INVALID, "streaming");
     if (p->precision < TDLB_PREC_F32 || p->precision > TDLB_PREC_BF16)
@@ -487,7 +488,7 @@ int tdlb_cu_initialise(tdlb_cu_t cu, double rho) {
     } else {
         const int nq = cu->simple() ? 2 : 1;
         for (int b = 0; b < nq; b++)
-            HIPCHK(launch_fill_q(cu->prec(), cu->prm.layout, cu->Q[b], cu->g, r, cu->stream));
+            HIPCHK(launch_fill_q(cu->prec(), cu->dlay(), cu->Q[b], cu->g, r, cu->stream));
     }
     HIPCHK(launch_zero(cu->F, cu->f_bytes(), cu->stream));
     HIPCHK(launch_zero(cu->Nu, cu->nu_bytes(), cu->stream));
@@ -512,10 +513,10 @@ int tdlb_cu_collision(tdlb_cu_t cu) {
     const auto c = cu->cargs();
     void* q = cu->Q[cu->cur];
     if (cu->simple())
-        HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_COLL, q, q, q, cu->F, cu->Nu, cu->fmask[cu->fm],
+        HIPCHK(launch_simple(cu->prec(), cu->dlay(), MODE_COLL, q, q, q, cu->F, cu->Nu, cu->fmask[cu->fm],
                              cu->g, c, cu->rho0, cu->stream));
     else
-        HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_COLL, cu->even ? 1 : 0, q, q, cu->F, cu->Nu,
+        HIPCHK(launch_esotwist(cu->prec(), cu->dlay(), MODE_COLL, cu->even ? 1 : 0, q, q, cu->F, cu->Nu,
                                cu->fmask[cu->fm], cu->g, c, cu->rho0, cu->stream));
     return TDLB_OK;
 }
@@ -528,10 +529,10 @@ int tdlb_cu_moments(tdlb_cu_t cu) {
     const auto c = cu->cargs();
     void* q = cu->Q[cu->cur];
     if (cu->simple())
-        HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_MOM, q, q, q, cu->F, cu->Nu, cu->fmask[cu->fm], cu->g,
+        HIPCHK(launch_simple(cu->prec(), cu->dlay(), MODE_MOM, q, q, q, cu->F, cu->Nu, cu->fmask[cu->fm], cu->g,
                              c, cu->rho0, cu->stream));
     else
-        HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_MOM, cu->even ? 1 : 0, q, q, cu->F, cu->Nu,
+        HIPCHK(launch_esotwist(cu->prec(), cu->dlay(), MODE_MOM, cu->even ? 1 : 0, q, q, cu->F, cu->Nu,
                                cu->fmask[cu->fm], cu->g, c, cu->rho0, cu->stream));
     return TDLB_OK;
 }
@@ -545,7 +546,7 @@ int tdlb_cu_stream_push(tdlb_cu_t cu) {
         cu->even = !cu->even;
         return TDLB_OK;
     }
-    HIPCHK(launch_push_all((int)cu->tsize, cu->prm.layout, cu->Q[cu->cur], cu->Q[1 - cu->cur], cu->g, cu->stream));
+    HIPCHK(launch_push_all((int)cu->tsize, cu->dlay(), cu->Q[cu->cur], cu->Q[1 - cu->cur], cu->g, cu->stream));
     cu->cur = 1 - cu->cur;
     return TDLB_OK;
 }
@@ -559,7 +560,7 @@ int tdlb_cu_stream_pull(tdlb_cu_t cu) {
         cu->even = !cu->even;
         return TDLB_OK;
     }
-    HIPCHK(launch_pull_all((int)cu->tsize, cu->prm.layout, cu->Q[cu->cur], cu->Q[1 - cu->cur], cu->g, cu->stream));
+    HIPCHK(launch_pull_all((int)cu->tsize, cu->dlay(), cu->Q[cu->cur], cu->Q[1 - cu->cur], cu->g, cu->stream));
     cu->cur = 1 - cu->cur;
     return TDLB_OK;
 }
@@ -570,7 +571,7 @@ int tdlb_cu_bounce_back(tdlb_cu_t cu) {
     CHK(set_device(cu));
     CHK(materialise(cu));
     if (!cu->simple()) return TDLB_OK;  // Boundary.hpp:13-14
-    HIPCHK(launch_bounce_back((int)cu->tsize, cu->prm.layout, cu->Q[cu->cur], cu->g, cu->stream));
+    HIPCHK(launch_bounce_back((int)cu->tsize, cu->dlay(), cu->Q[cu->cur], cu->g, cu->stream));
     return TDLB_OK;
 }
 
@@ -631,6 +632,16 @@ int tdlb_cu_download(tdlb_cu_t cu, int field, void* host, size_t bytes) {
         if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("download F: ") + hipGetErrorString(e));
         return TDLB_OK;
     }
+    if (field == TDLB_FIELD_Q && cu->blay() != cu->dlay()) {
+        void* tmp = nullptr;
+        if (hipMalloc(&tmp, need) != hipSuccess) return fail(TDLB_ERR_NOMEM, "download Q staging");
+        hipError_t e = launch_relayout_q((int)cu->tsize, cu->blay(), cu->eq(), tmp, cu->g, cu->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(host, tmp, need, hipMemcpyDeviceToHost, cu->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(cu->stream);
+        (void)hipFree(tmp);
+        if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("download Q: ") + hipGetErrorString(e));
+        return TDLB_OK;
+    }
     const void* src = field == TDLB_FIELD_Q ? cu->eq() : field == TDLB_FIELD_NU ? cu->Nu : (const void*)cu->O;
     HIPCHK(hipMemcpyAsync(host, src, need, hipMemcpyDeviceToHost, cu->stream));
     HIPCHK(hipStreamSynchronize(cu->stream));
@@ -645,6 +656,16 @@ int tdlb_cu_upload(tdlb_cu_t cu, int field, const void* host, size_t bytes) {
     CHK(materialise(cu));
     void* dst = field == TDLB_FIELD_Q ? cu->eq() : field == TDLB_FIELD_F ? cu->F
               : field == TDLB_FIELD_NU ? cu->Nu : (void*)cu->O;
+    if (field == TDLB_FIELD_Q && cu->blay() != cu->dlay()) {
+        void* tmp = nullptr;
+        if (hipMalloc(&tmp, need) != hipSuccess) return fail(TDLB_ERR_NOMEM, "upload Q staging");
+        hipError_t e = hipMemcpyAsync(tmp, host, need, hipMemcpyHostToDevice, cu->stream);
+        if (e == hipSuccess) e = launch_relayout_q((int)cu->tsize, cu->dlay(), tmp, dst, cu->g, cu->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(cu->stream);
+        (void)hipFree(tmp);
+        if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("upload Q: ") + hipGetErrorString(e));
+        return TDLB_OK;
+    }
     HIPCHK(hipMemcpyAsync(dst, host, need, hipMemcpyHostToDevice, cu->stream));
     if (field == TDLB_FIELD_F) {
         HIPCHK(launch_mask_from_f(cu->prec(), cu->F, cu->fmask[cu->fm], cu->gmask, cu->g, cu->stream));
@@ -921,25 +942,25 @@ int tdlb_cu_run(tdlb_cu_t cu, int64_t nsteps, uint64_t* step, double* angle) {
             void* e = cu->eq();
             if (cu->simple()) {
                 // settle the ghost layer once (bounce-back rims); the 16-bit lattices never read theirs
-                HIPCHK(launch_bounce_back((int)cu->tsize, cu->prm.layout, e, cu->g, cu->stream));
+                HIPCHK(launch_bounce_back((int)cu->tsize, cu->dlay(), e, cu->g, cu->stream));
                 if (!cu->shifted())
-                    HIPCHK(launch_ghost_copy((int)cu->ssize, cu->prm.layout, q, cu->Q[1 - cu->cur], cu->g,
+                    HIPCHK(launch_ghost_copy((int)cu->ssize, cu->dlay(), q, cu->Q[1 - cu->cur], cu->g,
                                              cu->stream));
-                HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_COLL, q, q, e, cu->F, cu->Nu,
+                HIPCHK(launch_simple(cu->prec(), cu->dlay(), MODE_COLL, q, q, e, cu->F, cu->Nu,
                                      cu->fmask[cu->fm], cu->g, c, cu->rho0, cu->stream));
             } else {
-                HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_COLL, cu->even ? 1 : 0, q, e, cu->F, cu->Nu,
+                HIPCHK(launch_esotwist(cu->prec(), cu->dlay(), MODE_COLL, cu->even ? 1 : 0, q, e, cu->F, cu->Nu,
                                        cu->fmask[cu->fm], cu->g, c, cu->rho0, cu->stream));
                 cu->even = !cu->even;
             }
             cu->phase = PH_LAZY;
         } else {
             if (cu->simple()) {
-                HIPCHK(launch_simple(cu->prec(), cu->prm.layout, MODE_FUSED, cu->Q[cu->cur], cu->Q[1 - cu->cur],
+                HIPCHK(launch_simple(cu->prec(), cu->dlay(), MODE_FUSED, cu->Q[cu->cur], cu->Q[1 - cu->cur],
                                      nullptr, cu->F, cu->Nu, cu->fmask[cu->fm], cu->g, c, cu->rho0, cu->stream));
                 cu->cur = 1 - cu->cur;
             } else {
-                HIPCHK(launch_esotwist(cu->prec(), cu->prm.layout, MODE_FUSED, cu->even ? 1 : 0, cu->Q[0], nullptr,
+                HIPCHK(launch_esotwist(cu->prec(), cu->dlay(), MODE_FUSED, cu->even ? 1 : 0, cu->Q[0], nullptr,
                                        cu->F, cu->Nu, cu->fmask[cu->fm], cu->g, c, cu->rho0, cu->stream));
                 cu->even = !cu->even;
             }
@@ -1055,7 +1076,7 @@ int tdlb_cu_reductions(tdlb_cu_t cu, double* out) {
     }
     const CellSource cs = cell_source(cu);
     double* res = cu->red + (size_t)kReduceBlocks * kReduceVals;
-    HIPCHK(launch_reduce(cs.prec, cu->prm.layout, cu->prm.streaming, cs.momsrc, cs.even, cs.Q, cu->F,
+    HIPCHK(launch_reduce(cs.prec, cu->dlay(), cu->prm.streaming, cs.momsrc, cs.even, cs.Q, cu->F,
                          cu->fmask[cu->fm], cu->g, ra, cu->rho0, cu->red, res, cu->stream));
     HIPCHK(hipMemcpyAsync(out, res, sizeof(double) * kReduceVals, hipMemcpyDeviceToHost, cu->stream));
     HIPCHK(hipStreamSynchronize(cu->stream));
@@ -1070,7 +1091,7 @@ int tdlb_cu_macroscopic(tdlb_cu_t cu, void* out, size_t bytes) {
     void* tmp = nullptr;
     if (hipMalloc(&tmp, need) != hipSuccess) return fail(TDLB_ERR_NOMEM, "macroscopic staging");
     const CellSource cs = cell_source(cu);
-    hipError_t e = launch_macroscopic(cs.prec, cu->prm.layout, cu->prm.streaming, cs.momsrc, cs.even, cs.Q, cu->F,
+    hipError_t e = launch_macroscopic(cs.prec, cu->dlay(), cu->prm.streaming, cs.momsrc, cs.even, cs.Q, cu->F,
                                       cu->fmask[cu->fm], cu->g, cu->rho0, tmp, cu->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, need, hipMemcpyDeviceToHost, cu->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(cu->stream);

@@ -157,6 +157,10 @@ struct tdlb_cu {
     size_t nu_bytes() const { return (size_t)g.size * tsize; }
     size_t mask_bytes() const { return mask_words * sizeof(uint64_t); }
     bool simple() const { return prm.streaming == TDLB_STREAM_SIMPLE; }
+    // layout of the lattice in HBM (what the kernels address) and of Q at the
+    // boundary (download, upload, checkpoint); they differ for TDLB_LAYOUT_IJKL_SOA
+    int dlay() const { return prm.layout == TDLB_LAYOUT_IJKL ? LayoutIJKL : LayoutLIJK; }
+    int blay() const { return prm.layout == TDLB_LAYOUT_LIJK ? LayoutLIJK : LayoutIJKL; }
     bool shifted() const { return prm.precision == TDLB_PREC_F16 || prm.precision == TDLB_PREC_BF16; }
     int prec() const { return prm.precision; }            // kernels on the stored populations
     int eprec() const { return prm.precision == TDLB_PREC_F64 ? 1 : 0; }  // kernels on the explicit Q

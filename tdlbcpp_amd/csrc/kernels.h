@@ -77,6 +77,8 @@ hipError_t launch_push_all(int esize, int layout, const void* src, void* dst, co
 hipError_t launch_pull_all(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s);
 hipError_t launch_bounce_back(int esize, int layout, void* Q, const Grid& g, hipStream_t s);
 hipError_t launch_ghost_fill(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s);
+// Q (size x 18 elements of esize bytes, 4 or 8) written in layout `to` from the other layout
+hipError_t launch_relayout_q(int esize, int to, const void* src, void* dst, const Grid& g, hipStream_t s);
 // ghost fill from reduced-precision populations (prec 2/3) into a float array
 hipError_t launch_ghost_fill_decode(int prec, int layout, const void* src, void* dst, const Grid& g, float rho0,
                                     hipStream_t s);

@@ -93,6 +93,25 @@ __global__ void k_fill_q(T* __restrict__ Q, Grid g, T rho) {
     }
 }
 
+// one destination element per thread: IJKL slot n*18 + l <-> LIJK slot l*size + n
+template <typename W, int TO>
+__global__ void k_relayout_q(const W* __restrict__ src, W* __restrict__ dst, Grid g) {
+    const uint64_t total = g.size * (uint64_t)kQ;
+    for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < total;
+         a += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t n, l;
+        if (TO == LayoutLIJK) {
+            l = a / g.size;
+            n = a - l * g.size;
+            dst[a] = src[n * kQ + l];
+        } else {
+            n = a / kQ;
+            l = a - n * kQ;
+            dst[a] = src[l * g.size + n];
+        }
+    }
+}
+
 __global__ void k_zero_words(uint32_t* __restrict__ p, uint64_t n) {
     for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < n; a += (uint64_t)gridDim.x * blockDim.x)
         p[a] = 0u;
@@ -1044,6 +1063,24 @@ hipError_t launch_fill_q(int prec, int layout, void* Q, const Grid& g, double rh
     } else {
         if (layout == LayoutIJKL) hipLaunchKernelGGL((k_fill_q<double, LayoutIJKL>), dim3(nb), dim3(kBlock), 0, s, (double*)Q, g, rho);
         else hipLaunchKernelGGL((k_fill_q<double, LayoutLIJK>), dim3(nb), dim3(kBlock), 0, s, (double*)Q, g, rho);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_relayout_q(int esize, int to, const void* src, void* dst, const Grid& g, hipStream_t s) {
+    const uint32_t nb = grid_blocks(g.size * kQ);
+    if (esize == 4) {
+        if (to == LayoutLIJK)
+            hipLaunchKernelGGL((k_relayout_q<uint32_t, LayoutLIJK>), dim3(nb), dim3(kBlock), 0, s, (const uint32_t*)src, (uint32_t*)dst, g);
+        else
+            hipLaunchKernelGGL((k_relayout_q<uint32_t, LayoutIJKL>), dim3(nb), dim3(kBlock), 0, s, (const uint32_t*)src, (uint32_t*)dst, g);
+    } else if (esize == 8) {
+        if (to == LayoutLIJK)
+            hipLaunchKernelGGL((k_relayout_q<uint64_t, LayoutLIJK>), dim3(nb), dim3(kBlock), 0, s, (const uint64_t*)src, (uint64_t*)dst, g);
+        else
+            hipLaunchKernelGGL((k_relayout_q<uint64_t, LayoutIJKL>), dim3(nb), dim3(kBlock), 0, s, (const uint64_t*)src, (uint64_t*)dst, g);
+    } else {
+        return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -29,7 +29,8 @@ def case(msg):
 
 
 @pytest.mark.parametrize("streaming", [td.STREAM_SIMPLE, td.STREAM_ESOTWIST], ids=["Simple", "Esotwist"])
-@pytest.mark.parametrize("layout", [td.LAYOUT_IJKL, td.LAYOUT_LIJK], ids=["IJKL", "LIJK"])
+@pytest.mark.parametrize("layout", [td.LAYOUT_IJKL, td.LAYOUT_LIJK, td.LAYOUT_IJKL_SOA],
+                         ids=["IJKL", "LIJK", "IJKL_SOA"])
 def test_checkpoint_write_read_round_trip(tmp_path, streaming, layout):
     case(f"checkpoint round trip {streaming} {layout}")
     T = 36

@@ -16,7 +16,9 @@ from tests.helpers import ReferenceLoop, assert_close_moments, flow_for_tank, ra
 
 pytestmark = pytest.mark.gpu
 
-LAYOUTS = [(td.LAYOUT_IJKL, O.IJKL), (td.LAYOUT_LIJK, O.LIJK)]
+# IJKL_SOA keeps the lattice LIJK in HBM and IJKL at the boundary: it must match the IJKL oracle
+LAYOUTS = [(td.LAYOUT_IJKL, O.IJKL), (td.LAYOUT_LIJK, O.LIJK), (td.LAYOUT_IJKL_SOA, O.IJKL)]
+LAYOUT_IDS = ["IJKL", "LIJK", "IJKL_SOA"]
 STREAMS = [(td.STREAM_SIMPLE, O.SIMPLE), (td.STREAM_ESOTWIST, O.ESOTWIST)]
 
 
@@ -35,7 +37,7 @@ def tag_pair(layout, x=3, y=3, z=3):
 
 
 # ---------------------------------------------------------------- data movement
-@pytest.mark.parametrize("layout", LAYOUTS, ids=["IJKL", "LIJK"])
+@pytest.mark.parametrize("layout", LAYOUTS, ids=LAYOUT_IDS)
 def test_gpu_push_matches_reference_fixture(layout, golden_dir):
     case(f"push golden {layout}")
     _, g = tag_pair(layout)
@@ -44,7 +46,7 @@ def test_gpu_push_matches_reference_fixture(layout, golden_dir):
     np.testing.assert_array_equal(got, np.load(f"{golden_dir}/streaming_push_Q.npy"))
 
 
-@pytest.mark.parametrize("layout", LAYOUTS, ids=["IJKL", "LIJK"])
+@pytest.mark.parametrize("layout", LAYOUTS, ids=LAYOUT_IDS)
 def test_gpu_pull_matches_reference_fixture(layout, golden_dir):
     case(f"pull golden {layout}")
     _, g = tag_pair(layout)
@@ -52,7 +54,7 @@ def test_gpu_pull_matches_reference_fixture(layout, golden_dir):
     np.testing.assert_array_equal(g.q_ijkl(dtype=np.uint64), np.load(f"{golden_dir}/streaming_pull_Q.npy"))
 
 
-@pytest.mark.parametrize("layout", LAYOUTS, ids=["IJKL", "LIJK"])
+@pytest.mark.parametrize("layout", LAYOUTS, ids=LAYOUT_IDS)
 @pytest.mark.parametrize("op", ["push", "pull", "bounce_back", "bb_then_push"])
 def test_gpu_data_movement_matches_oracle_nonsquare(layout, op):
     case(f"data movement {op} {layout}")
@@ -90,7 +92,7 @@ def seeded_pair(layout, stream, prec, les, shape=(7, 6, 9), seed=1, even=True):
     return o, g
 
 
-@pytest.mark.parametrize("layout", LAYOUTS, ids=["IJKL", "LIJK"])
+@pytest.mark.parametrize("layout", LAYOUTS, ids=LAYOUT_IDS)
 @pytest.mark.parametrize("stream", STREAMS, ids=["Simple", "Esotwist"])
 @pytest.mark.parametrize("prec", [td.PREC_F32, td.PREC_F64], ids=["f32", "f64"])
 @pytest.mark.parametrize("les", [False, True], ids=["noLES", "LES"])
@@ -121,7 +123,7 @@ def random_points(rng, shape, n):
     return pts
 
 
-@pytest.mark.parametrize("layout", LAYOUTS, ids=["IJKL", "LIJK"])
+@pytest.mark.parametrize("layout", LAYOUTS, ids=LAYOUT_IDS)
 @pytest.mark.parametrize("stream", STREAMS, ids=["Simple", "Esotwist"])
 def test_gpu_forcing_bit_exact_with_overlaps_and_wrap(layout, stream):
     case(f"forcing {layout} {stream}")
@@ -142,7 +144,7 @@ def test_gpu_forcing_bit_exact_with_overlaps_and_wrap(layout, stream):
 
 
 # ---------------------------------------------------------------- main loop
-@pytest.mark.parametrize("layout", LAYOUTS, ids=["IJKL", "LIJK"])
+@pytest.mark.parametrize("layout", LAYOUTS, ids=LAYOUT_IDS)
 @pytest.mark.parametrize("stream", STREAMS, ids=["Simple", "Esotwist"])
 def test_gpu_main_loop_20_steps_matches_oracle(layout, stream):
     T = 36  # smallest tank above 30 whose blades pass the on-device colouring check
