@@ -289,7 +289,8 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build runs one ComputeUnit on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU slab decomposition is designed, not built: see DESIGN.md)", file=sys.stderr)
+              "(one GPU runs one tank or a batch of tanks; a slab decomposition over GPUs is not built)",
+              file=sys.stderr)
         return 2
     if args.copy_calibration > 0:
         return copy_calibration(args.copy_calibration)

@@ -68,8 +68,18 @@ def lib():
         L.orc_weight_f64.restype = d
         L.orc_moments_of_f64.argtypes = [vp, vp]
         L.orc_moments_of_f32.argtypes = [vp, vp]
+        L.orc_double_resolution.argtypes = [vp]
+        L.orc_plane_xz.argtypes = [vp, i64, vp, i64, i64, vp, ctypes.POINTER(i64), vp, ctypes.POINTER(i64)]
+        L.orc_vorticity.argtypes = [vp, ctypes.c_int, i64, vp, vp]
         _lib = L
     return _lib
+
+
+# savePlaneXZ records (Output.hpp:22-25: tDiskGrid<float, 4> and <float, 3>)
+PLANE_Q_DTYPE = np.dtype({"names": ["i", "j", "k", "q"], "formats": ["<u2", "<u2", "<u2", ("<f4", 4)],
+                          "offsets": [0, 2, 4, 8], "itemsize": 24})
+PLANE_F_DTYPE = np.dtype({"names": ["i", "j", "k", "q"], "formats": ["<u2", "<u2", "<u2", ("<f4", 3)],
+                          "offsets": [0, 2, 4, 8], "itemsize": 20})
 
 
 class OracleCU:
@@ -116,6 +126,30 @@ class OracleCU:
     def forcing(self, points, alfa, beta):
         pts = np.ascontiguousarray(points, dtype=POINT_DTYPE)
         lib().orc_forcing(self._h, pts.ctypes.data if len(pts) else None, len(pts), alfa, beta)
+
+    def double_resolution(self):
+        """doubleResolutionFullCU: the grid doubles in x, y, z."""
+        lib().orc_double_resolution(self._h)
+        self.xg, self.yg, self.zg, self.size = (lib().orc_dim(self._h, w) for w in range(4))
+
+    def plane_xz(self, cut, exclude=None, i0=0, k0=0):
+        """savePlaneXZ records: (QVec records, F3 records) as structured arrays."""
+        n = (self.xg - 2) * (self.zg - 2)
+        q = np.zeros(n, dtype=PLANE_Q_DTYPE)
+        f = np.zeros(n, dtype=PLANE_F_DTYPE)
+        nq, nf = ctypes.c_int64(), ctypes.c_int64()
+        ex = None if exclude is None else np.ascontiguousarray(exclude, dtype=np.uint8)
+        lib().orc_plane_xz(self._h, cut, None if ex is None else ex.ctypes.data, i0, k0, q.ctypes.data,
+                           ctypes.byref(nq), f.ctypes.data, ctypes.byref(nf))
+        return q[:nq.value], f[:nf.value]
+
+    def vorticity(self, axis, cut, exclude=None):
+        """log |curl u|^2 on plane j = cut (axis 0, shape (x, z)) or k = cut
+        (axis 1, shape (x, y)); NaN where excluded."""
+        out = np.zeros((self.xg - 2, (self.zg if axis == 0 else self.yg) - 2), dtype=self.np_dtype)
+        ex = None if exclude is None else np.ascontiguousarray(exclude, dtype=np.uint8)
+        lib().orc_vorticity(self._h, axis, cut, None if ex is None else ex.ctypes.data, out.ctypes.data)
+        return out
 
     def interior_shape(self):
         g = 2 * self.ghost

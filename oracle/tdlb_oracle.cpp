@@ -14,6 +14,9 @@
 //   bounce-back            tdlbcpp/src/Boundary.hpp:13-250
 //   IB forcing             tdlbcpp/src/Forcing.hpp:18-176
 //   calcNuAndRe_m          tdlbcpp/src/Params/FlowParams.hpp:65-70
+//   doubleResolutionFullCU tdlbcpp/src/ComputeUnit.hpp:229-316,480-502
+//   savePlaneXZ records    tdlbcpp/src/ComputeUnitOutput.hpp:354-447
+//   calcVorticityXZ field  tdlbcpp/src/ComputeUnitOutput.hpp:113-160
 //
 // Arithmetic keeps the reference's literal types: in the float build the
 // double literals (0.5, 1.0, 2.0, 1.5, 24.0, ...) promote sub-expressions to
@@ -21,13 +24,14 @@
 // added in the reference's order.  Build with -ffp-contract=off so that no
 // multiply-add is fused.
 //
-// Parity status of each function (how it is pinned) is listed in DESIGN.md's
-// oracle table; the tests that pin it live in tests/test_oracle*.py.
+// The checks that pin each function against the reference's fixtures and the
+// mathematics live in tests/test_oracle.py.
 
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <cstdio>
+#include <type_traits>
 #include <vector>
 #include <algorithm>
 
@@ -71,6 +75,11 @@ struct Base {
     virtual bool& even() = 0;
     virtual int64_t dims(int which) = 0;
     virtual void cells(int op, void* buf) = 0;
+    virtual void doubleResolution() = 0;
+    virtual void planeXZ(int64_t cut, const uint8_t* exclude, int64_t i0, int64_t k0, void* q, int64_t* nq, void* f,
+                         int64_t* nf) = 0;
+    virtual void vorticityXZ(int64_t cut, const uint8_t* exclude, void* out) = 0;
+    virtual void vorticityXY(int64_t cut, const uint8_t* exclude, void* out) = 0;
 };
 
 template <typename T, int LAYOUT, int STREAM>
@@ -458,6 +467,124 @@ struct CU : Base {
                 }
     }
 
+    // doubleResolutionFullCU + copyFieldsWithScaling + indexPreviousResolution
+    // (ComputeUnit.hpp:229-316, 480-502): every cell of the doubled grid, ghosts
+    // included, takes Q and Nu of old cell (i/2 + i%2, j/2 + j%2, k/2 + k%2);
+    // F, O are zero (initialise(0)).  The reference copies 19 slots of an
+    // 18-slot record; the 19th write lands in the next cell's slot 0 (IJKL,
+    // rewritten by that cell's own copy) or past the array (LIJK), so inside
+    // the array the result is the 18-slot copy done here.
+    void doubleResolution() override {
+        const int64_t f = 2;
+        const std::vector<T> oQ = Q, oNu = Nu;
+        const size_t osize = size;
+        x *= f; y *= f; z *= f;
+        xg = x + 2 * ghost; yg = y + 2 * ghost; zg = z + 2 * ghost;
+        xg0 = xg - 1; yg0 = yg - 1; zg0 = zg - 1;
+        xg1 = xg - 2; yg1 = yg - 2; zg1 = zg - 2;
+        size = size_t(xg) * yg * zg;
+        Q.assign(size * 18, T(0));
+        F.assign(size * 3, T(0));
+        Nu.assign(size, T(0));
+        O.assign(size, 0);
+        const int64_t fromYG = y / f + ghost * 2, fromZG = z / f + ghost * 2;
+        for (int64_t i = 0; i <= xg0; i++)
+            for (int64_t j = 0; j <= yg0; j++)
+                for (int64_t k = 0; k <= zg0; k++) {
+                    const int64_t fi = i / f + i % f, fj = j / f + j % f, fk = k / f + k % f;
+                    const size_t from = size_t(fi * (fromYG * fromZG) + fj * fromZG + fk);
+                    const int64_t n = index(i, j, k);
+                    for (int l = 0; l < 18; l++)
+                        slot(n, l) = LAYOUT == IJKL ? oQ[from * 18 + l] : oQ[size_t(l) * osize + from];
+                    Nu[n] = oNu[from];
+                }
+    }
+
+    // savePlaneXZ<float, 4> (ComputeUnitOutput.hpp:354-447): records of the
+    // cells of plane j = cut (i, k interior, i-major) that are not excluded:
+    // {i0+i-1, j-1, k0+k-1} and the first 4 raw slots of the cell (the storage
+    // at index(i,j,k), not the streaming accessor); F records where F is not
+    // "not zero" (Force::isNotZero, QVec.hpp:54-56) means all three components
+    // non-zero, carry {i0+i-1, j, k0+k-1} (no -1 on j, as the reference writes it).
+    struct Rec4 { uint16_t i, j, k; float q[4]; };
+    struct Rec3 { uint16_t i, j, k; float q[3]; };
+    void planeXZ(int64_t cut, const uint8_t* exclude, int64_t i0, int64_t k0, void* qv, int64_t* nq, void* fv,
+                 int64_t* nf) override {
+        Rec4* q = static_cast<Rec4*>(qv);
+        Rec3* fr = static_cast<Rec3*>(fv);
+        int64_t a = 0, b = 0;
+        for (int64_t i = 1; i <= xg1; i++)
+            for (int64_t k = 1; k <= zg1; k++) {
+                const int64_t n = index(i, cut, k);
+                if (exclude && exclude[n]) continue;
+                Rec4 r{};
+                r.i = uint16_t(i0 + i - 1); r.j = uint16_t(cut - 1); r.k = uint16_t(k0 + k - 1);
+                for (int l = 0; l < 4; l++) r.q[l] = float(slot(n, l));
+                q[a++] = r;
+                if (F[3 * n] != 0.0 && F[3 * n + 1] != 0.0 && F[3 * n + 2] != 0.0) {
+                    Rec3 s{};
+                    s.i = uint16_t(i0 + i - 1); s.j = uint16_t(cut); s.k = uint16_t(k0 + k - 1);
+                    for (int l = 0; l < 3; l++) s.q[l] = float(F[3 * n + l]);
+                    fr[b++] = s;
+                }
+            }
+        *nq = a;
+        *nf = b;
+    }
+
+    // calcVorticityXZ / calcVorticityXY (ComputeUnitOutput.hpp:113-160,
+    // 198-245) before the picture is made: log|curl u|^2 from central
+    // differences of QVec::velocity() (no force term) of the accessor reads
+    // around the cell; excluded cells NaN.  axis 0: plane j = cut, x*z values
+    // (i-major); axis 1: plane k = cut, x*y values (i-major).
+    void vorticityXZ(int64_t cut, const uint8_t* exclude, void* outv) override {
+        if constexpr (std::is_floating_point<T>::value) vorticity(0, cut, exclude, static_cast<T*>(outv));
+    }
+    void vorticityXY(int64_t cut, const uint8_t* exclude, void* outv) override {
+        if constexpr (std::is_floating_point<T>::value) vorticity(1, cut, exclude, static_cast<T*>(outv));
+    }
+    void vorticity(int axis, int64_t cut, const uint8_t* exclude, T* out) {
+        T qa[18], qb[18];
+        auto vel = [&](int64_t i, int64_t j, int64_t k, T& ux, T& uy, T& uz) {
+            // the twisted read of a far-side ghost cell leaves the array in the
+            // reference (home = ghost + 1): undefined there, NaN here
+            if (STREAM == ESOTWIST && (i == xg1 + 1 || j == yg1 + 1 || k == zg1 + 1)) {
+                ux = uy = uz = T(NAN);
+                return;
+            }
+            readCell(i, j, k, qa);
+            ux = qa[M02] / qa[M01];
+            uy = qa[M03] / qa[M01];
+            uz = qa[M04] / qa[M01];
+        };
+        (void)qb;
+        const int64_t nb = axis == 0 ? zg1 : yg1;
+        for (int64_t i = 1; i <= xg1; i++)
+            for (int64_t c = 1; c <= nb; c++) {
+                const int64_t j = axis == 0 ? cut : c, k = axis == 0 ? c : cut;
+                T* o = &out[(i - 1) * nb + (c - 1)];
+                const int64_t n = index(i, j, k);
+                if (exclude && exclude[n]) { *o = T(NAN); continue; }
+                T x5, y5, z5, x6, y6, z6, x3, y3, z3, x4, y4, z4, x1, y1, z1, x2, y2, z2;
+                vel(i, j, k + 1, x5, y5, z5);
+                vel(i, j, k - 1, x6, y6, z6);
+                const T uxy = T(0.5) * (x5 - x6);
+                vel(i, j + 1, k, x3, y3, z3);
+                vel(i, j - 1, k, x4, y4, z4);
+                const T uxz = T(0.5) * (x3 - x4);
+                vel(i + 1, j, k, x1, y1, z1);
+                vel(i - 1, j, k, x2, y2, z2);
+                const T uyx = T(0.5) * (y1 - y2);
+                const T uyz = T(0.5) * (y3 - y4);
+                const T uzx = T(0.5) * (z1 - z2);
+                const T uzy = T(0.5) * (z5 - z6);
+                const T uxyuyx = uxy - uyx;
+                const T uyzuzy = uyz - uzy;
+                const T uzxuxz = uzx - uxz;
+                *o = T(std::log(T(uyzuzy * uyzuzy + uzxuxz * uzxuxz + uxyuyx * uxyuyx)));
+            }
+    }
+
     void* field(int which, size_t* bytes) override {
         switch (which) {
             case 0: *bytes = Q.size() * sizeof(T); return Q.data();
@@ -547,6 +674,16 @@ void orc_set(void* h, int field, const void* in) {
 int orc_even_step(void* h) { return static_cast<orc::Base*>(h)->even() ? 1 : 0; }
 void orc_set_even_step(void* h, int e) { static_cast<orc::Base*>(h)->even() = (e != 0); }
 int64_t orc_dim(void* h, int which) { return static_cast<orc::Base*>(h)->dims(which); }
+void orc_double_resolution(void* h) { static_cast<orc::Base*>(h)->doubleResolution(); }
+// records of 24 bytes {u16 i, j, k; f32 q[4]} and 20 bytes {u16 i, j, k; f32 q[3]}; x*z capacity each
+void orc_plane_xz(void* h, int64_t cut, const uint8_t* exclude, int64_t i0, int64_t k0, void* q, int64_t* nq,
+                  void* f, int64_t* nf) {
+    static_cast<orc::Base*>(h)->planeXZ(cut, exclude, i0, k0, q, nq, f, nf);
+}
+void orc_vorticity(void* h, int axis, int64_t cut, const uint8_t* exclude, void* out) {
+    if (axis == 0) static_cast<orc::Base*>(h)->vorticityXZ(cut, exclude, out);
+    else static_cast<orc::Base*>(h)->vorticityXY(cut, exclude, out);
+}
 void orc_cells(void* h, int op, void* buf) { static_cast<orc::Base*>(h)->cells(op, buf); }
 
 // calcNuAndRe_m (FlowParams.hpp:65-70) in float: returns nu, writes Re_m.

@@ -24,6 +24,13 @@ POINT_DTYPE = np.dtype([("i", "<i8"), ("j", "<i8"), ("k", "<i8"), ("i_frac", "<f
                         ("u_delta", "<f8"), ("v_delta", "<f8"), ("w_delta", "<f8")])
 
 
+# savePlaneXZ records (tDiskGrid<float, 4> and <float, 3>)
+PLANE_Q_DTYPE = np.dtype({"names": ["i", "j", "k", "q"], "formats": ["<u2", "<u2", "<u2", ("<f4", 4)],
+                          "offsets": [0, 2, 4, 8], "itemsize": 24})
+PLANE_F_DTYPE = np.dtype({"names": ["i", "j", "k", "q"], "formats": ["<u2", "<u2", "<u2", ("<f4", 3)],
+                          "offsets": [0, 2, 4, 8], "itemsize": 20})
+
+
 class CuParams(ctypes.Structure):
     _fields_ = [("x", ctypes.c_int64), ("y", ctypes.c_int64), ("z", ctypes.c_int64), ("ghost", ctypes.c_int64),
                 ("i0", ctypes.c_int64), ("j0", ctypes.c_int64), ("k0", ctypes.c_int64),
@@ -110,6 +117,10 @@ def lib():
             "tdlb_cu_profile_read": ([vp, P(d), P(i64), P(d), P(i64)], i32),
             "tdlb_cu_reductions": ([vp, P(d)], i32),
             "tdlb_cu_macroscopic": ([vp, vp, ctypes.c_size_t], i32),
+            "tdlb_cu_double_resolution": ([vp], i32),
+            "tdlb_cu_output_exclude": ([vp, vp, ctypes.c_size_t], i32),
+            "tdlb_cu_plane_xz": ([vp, i64, vp, vp, P(i64), vp, P(i64)], i32),
+            "tdlb_cu_vorticity": ([vp, ctypes.c_int32, i64, vp, vp, ctypes.c_size_t], i32),
             "tdlb_batch_create": ([i32, i32, i32, i32, P(vp)], i32),
             "tdlb_batch_destroy": ([vp], i32),
             "tdlb_batch_set_options": ([vp, i32, i32], i32),
@@ -355,6 +366,37 @@ class ComputeUnit:
         a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         check(lib().tdlb_cu_forcing_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def double_resolution(self):
+        """doubleResolutionFullCU: the grid doubles; bind the tank of the new size before run()."""
+        check(lib().tdlb_cu_double_resolution(self._h))
+        xg, yg, zg, size = (ctypes.c_int64() for _ in range(4))
+        check(lib().tdlb_cu_dims(self._h, ctypes.byref(xg), ctypes.byref(yg), ctypes.byref(zg), ctypes.byref(size)))
+        self.xg, self.yg, self.zg, self.size = xg.value, yg.value, zg.value, size.value
+
+    def output_exclude(self):
+        out = np.empty(self.size, dtype=np.uint8)
+        check(lib().tdlb_cu_output_exclude(self._h, out.ctypes.data, out.nbytes))
+        return out
+
+    def plane_xz(self, cut, exclude=None):
+        """savePlaneXZ records: (QVec records, F3 records) as structured arrays."""
+        n = (self.xg - 2) * (self.zg - 2)
+        q = np.zeros(n, dtype=PLANE_Q_DTYPE)
+        f = np.zeros(n, dtype=PLANE_F_DTYPE)
+        nq, nf = ctypes.c_int64(), ctypes.c_int64()
+        ex = None if exclude is None else np.ascontiguousarray(exclude, dtype=np.uint8)
+        check(lib().tdlb_cu_plane_xz(self._h, cut, None if ex is None else ex.ctypes.data, q.ctypes.data,
+                                     ctypes.byref(nq), f.ctypes.data, ctypes.byref(nf)))
+        return q[:nq.value], f[:nf.value]
+
+    def vorticity(self, axis, cut, exclude=None):
+        """log |curl u|^2 on plane j = cut (axis 0, (x, z)) or k = cut (axis 1, (x, y))."""
+        out = np.empty((self.xg - 2, (self.zg if axis == 0 else self.yg) - 2), dtype=self.np_dtype)
+        ex = None if exclude is None else np.ascontiguousarray(exclude, dtype=np.uint8)
+        check(lib().tdlb_cu_vorticity(self._h, axis, cut, None if ex is None else ex.ctypes.data, out.ctypes.data,
+                                      out.nbytes))
+        return out
 
     def reductions(self):
         """FP64 device sums: {"mass", "momentum" (3), "force" (3), "torque"}."""

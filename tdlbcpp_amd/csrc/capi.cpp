@@ -1100,6 +1100,138 @@ int tdlb_cu_macroscopic(tdlb_cu_t cu, void* out, size_t bytes) {
     return TDLB_OK;
 }
 
+int tdlb_cu_double_resolution(tdlb_cu_t cu) {
+    if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
+    CHK(set_device(cu));
+    CHK(materialise(cu));
+    tdlb_cu_params np = cu->prm;
+    np.x *= 2;
+    np.y *= 2;
+    np.z *= 2;
+    np.resolution += 2;  // resolution += factor (ComputeUnit.hpp:262)
+    tdlb_cu* n = nullptr;
+    CHK(tdlb_cu_create(&np, &cu->flow, &n));
+    auto drop = [&](int rc) {
+        tdlb_cu_destroy(n);
+        return rc;
+    };
+    n->rho0 = cu->rho0;
+    n->even = cu->even;
+    n->alfa = cu->alfa;
+    n->beta = cu->beta;
+    if (n->shifted()) {
+        const int rc = ensure_m(n);
+        if (rc != TDLB_OK) return drop(rc);
+    }
+    hipError_t e = hipStreamSynchronize(cu->stream);
+    if (e == hipSuccess)
+        e = launch_regrid((int)cu->tsize, cu->dlay(), cu->eq(), n->eq(), cu->Nu, n->Nu, cu->g, n->g, n->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
+    if (e != hipSuccess) return drop(fail(TDLB_ERR_HIP, std::string("regrid: ") + hipGetErrorString(e)));
+    n->phase = PH_EXPLICIT;
+    // the handle keeps its address; the old arrays leave with n.  F, O and the
+    // masks start at zero and no geometry is bound (the caller binds the tank
+    // of the new size, as main.cpp:360-399 regenerates it)
+    std::swap(*cu, *n);
+    tdlb_cu_destroy(n);
+    return TDLB_OK;
+}
+
+int tdlb_cu_output_exclude(tdlb_cu_t cu, uint8_t* out, size_t bytes) {
+    if (!cu || !out) return fail(TDLB_ERR_INVALID, "null argument");
+    if (!cu->bound) return fail(TDLB_ERR_STATE, "no geometry bound");
+    if (bytes != cu->g.size) return fail(TDLB_ERR_INVALID, "exclude mask must be xg*yg*zg bytes");
+    CHK(tdlb_cu_solid_mask(cu, out, bytes));
+    const int64_t n = cu->n_static + cu->n_rot;
+    std::vector<int32_t> i(n), j(n), k(n);
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(i.data(), cu->pts.d.i, n * 4, hipMemcpyDeviceToHost, cu->stream));
+        HIPCHK(hipMemcpyAsync(j.data(), cu->pts.d.j, n * 4, hipMemcpyDeviceToHost, cu->stream));
+        HIPCHK(hipMemcpyAsync(k.data(), cu->pts.d.k, n * 4, hipMemcpyDeviceToHost, cu->stream));
+        HIPCHK(hipStreamSynchronize(cu->stream));
+    }
+    for (int64_t q = 0; q < n; q++) out[((uint64_t)i[q] * cu->yg + (uint64_t)j[q]) * cu->zg + (uint64_t)k[q]] = 1;
+    return TDLB_OK;
+}
+
+// upload an optional exclude mask (size bytes) into a device buffer
+static int upload_exclude(tdlb_cu* cu, const uint8_t* exclude, uint8_t** dev) {
+    *dev = nullptr;
+    if (!exclude) return TDLB_OK;
+    if (hipMalloc((void**)dev, cu->g.size) != hipSuccess) return fail(TDLB_ERR_NOMEM, "exclude mask");
+    const hipError_t e = hipMemcpyAsync(*dev, exclude, cu->g.size, hipMemcpyHostToDevice, cu->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(*dev);
+        *dev = nullptr;
+        return fail(TDLB_ERR_HIP, std::string("exclude mask: ") + hipGetErrorString(e));
+    }
+    return TDLB_OK;
+}
+
+int tdlb_cu_plane_xz(tdlb_cu_t cu, int64_t cut, const uint8_t* exclude, void* qvec, int64_t* n_qvec, void* f3,
+                     int64_t* n_f3) {
+    if (!cu || !qvec || !f3 || !n_qvec || !n_f3) return fail(TDLB_ERR_INVALID, "null argument");
+    if (cut < 1 || cut > cu->g.yg1) return fail(TDLB_ERR_INVALID, "cut must be an interior j");
+    CHK(set_device(cu));
+    CHK(materialise(cu));
+    const int64_t cells = (int64_t)cu->g.xg1 * cu->g.zg1;
+    uint8_t* ex = nullptr;
+    CHK(upload_exclude(cu, exclude, &ex));
+    PlaneRec4* dq = nullptr;
+    PlaneRec3* df = nullptr;
+    uint8_t* dflag = nullptr;
+    std::vector<PlaneRec4> hq(cells);
+    std::vector<PlaneRec3> hf(cells);
+    std::vector<uint8_t> hflag(cells);
+    hipError_t e = hipMalloc((void**)&dq, cells * sizeof(PlaneRec4));
+    if (e == hipSuccess) e = hipMalloc((void**)&df, cells * sizeof(PlaneRec3));
+    if (e == hipSuccess) e = hipMalloc((void**)&dflag, cells);
+    if (e == hipSuccess)
+        e = launch_plane_xz(cu->eprec(), cu->dlay(), cu->eq(), cu->F, cu->fmask[cu->fm], ex, cu->g, (int)cut,
+                            cu->prm.i0, cu->prm.k0, dq, df, dflag, cu->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hq.data(), dq, cells * sizeof(PlaneRec4), hipMemcpyDeviceToHost, cu->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hf.data(), df, cells * sizeof(PlaneRec3), hipMemcpyDeviceToHost, cu->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hflag.data(), dflag, cells, hipMemcpyDeviceToHost, cu->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(cu->stream);
+    for (void* p : {(void*)dq, (void*)df, (void*)dflag, (void*)ex})
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("plane: ") + hipGetErrorString(e));
+    int64_t a = 0, b = 0;
+    auto* oq = static_cast<PlaneRec4*>(qvec);
+    auto* of = static_cast<PlaneRec3*>(f3);
+    for (int64_t t = 0; t < cells; t++) {
+        if (hflag[t] & 1) oq[a++] = hq[t];
+        if (hflag[t] & 2) of[b++] = hf[t];
+    }
+    *n_qvec = a;
+    *n_f3 = b;
+    return TDLB_OK;
+}
+
+int tdlb_cu_vorticity(tdlb_cu_t cu, int32_t axis, int64_t cut, const uint8_t* exclude, void* out, size_t bytes) {
+    if (!cu || !out) return fail(TDLB_ERR_INVALID, "null argument");
+    if (axis != 0 && axis != 1) return fail(TDLB_ERR_INVALID, "axis: 0 (plane j = cut) or 1 (plane k = cut)");
+    const int64_t lim = axis == 0 ? cu->g.yg1 : cu->g.zg1;
+    if (cut < 1 || cut > lim) return fail(TDLB_ERR_INVALID, "cut must be an interior index");
+    const size_t need = (size_t)cu->g.xg1 * (axis == 0 ? cu->g.zg1 : cu->g.yg1) * cu->tsize;
+    if (bytes != need) return fail(TDLB_ERR_INVALID, "vorticity: buffer size must be " + std::to_string(need));
+    CHK(set_device(cu));
+    CHK(materialise(cu));  // ghost neighbours are read as the reference holds them
+    uint8_t* ex = nullptr;
+    CHK(upload_exclude(cu, exclude, &ex));
+    void* tmp = nullptr;
+    hipError_t e = hipMalloc(&tmp, need);
+    if (e == hipSuccess)
+        e = launch_vorticity(cu->eprec(), cu->dlay(), cu->prm.streaming, MOMSRC_STORED, cu->even ? 1 : 0, cu->eq(),
+                             cu->g, axis, (int)cut, ex, tmp, cu->rho0, cu->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, need, hipMemcpyDeviceToHost, cu->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(cu->stream);
+    if (tmp) (void)hipFree(tmp);
+    if (ex) (void)hipFree(ex);
+    if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("vorticity: ") + hipGetErrorString(e));
+    return TDLB_OK;
+}
+
 int tdlb_cu_forcing_info(tdlb_cu_t cu, int64_t* n_static, int64_t* n_rotating, int64_t* n_levels) {
     if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
     if (n_static) *n_static = cu->n_static;

@@ -126,6 +126,28 @@ hipError_t launch_macroscopic(int prec, int layout, int streaming, int momsrc, i
                               const void* F, const uint64_t* fmask, const Grid& g, float rho0, void* out,
                               hipStream_t s);
 
+// doubleResolutionFullCU: Q (esize 4/8) and a per-cell array (esize 4/8) of the
+// old grid og copied into the doubled grid ng
+hipError_t launch_regrid(int esize, int layout, const void* oldQ, void* newQ, const void* oldNu, void* newNu,
+                         const Grid& og, const Grid& ng, hipStream_t s);
+
+// records of savePlaneXZ<float, 4> / <float, 3> (tDiskGrid, Output.hpp:22-25)
+struct PlaneRec4 {
+    uint16_t i, j, k;
+    float q[4];
+};
+struct PlaneRec3 {
+    uint16_t i, j, k;
+    float q[3];
+};
+// one record slot and one flag byte per plane cell (x * z, i-major); prec 0/1 (explicit Q type)
+hipError_t launch_plane_xz(int prec, int layout, const void* Q, const void* F, const uint64_t* fmask,
+                           const uint8_t* exclude, const Grid& g, int cut, int64_t i0, int64_t k0, PlaneRec4* q,
+                           PlaneRec3* f, uint8_t* flags, hipStream_t s);
+// vorticity field of plane j = cut (axis 0) or k = cut (axis 1), T per cell
+hipError_t launch_vorticity(int prec, int layout, int streaming, int momsrc, int even, const void* Q, const Grid& g,
+                            int axis, int cut, const uint8_t* exclude, void* out, float rho0, hipStream_t s);
+
 hipError_t launch_gen_blades(int prec, const BladeLayout& bl, const BladeStep& st, const PointsDev& out,
                              int64_t offset, hipStream_t s);
 hipError_t launch_solid_mask(double cx, double cz, double radius, uint8_t* out, const Grid& g, int ghost,

@@ -908,6 +908,119 @@ __global__ __launch_bounds__(kBlock) void k_macroscopic(const typename PR::S* __
     }
 }
 
+// ------------------------------------------------------------------ regrid and output planes
+// doubleResolutionFullCU (ComputeUnit.hpp:229-316, 480-502): every cell of the
+// doubled grid, ghosts included, takes old cell (i/2 + i%2, j/2 + j%2, k/2 + k%2).
+__device__ __forceinline__ uint32_t regrid_source(const Grid& og, const Grid& ng, uint64_t n) {
+    int i, j, k;
+    decode_cell(ng, n, i, j, k);
+    return cell_index(og, i / 2 + (i & 1), j / 2 + (j & 1), k / 2 + (k & 1));
+}
+
+template <typename W, int L>
+__global__ void k_regrid_q(const W* __restrict__ old, W* __restrict__ nw, Grid og, Grid ng) {
+    for (uint64_t n = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; n < ng.size;
+         n += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t from = regrid_source(og, ng, n);
+#pragma unroll
+        for (int l = 0; l < kQ; l++) nw[qaddr<L>(ng, (uint32_t)n, l)] = old[qaddr<L>(og, from, l)];
+    }
+}
+
+template <typename W>
+__global__ void k_regrid_cells(const W* __restrict__ old, W* __restrict__ nw, Grid og, Grid ng) {
+    for (uint64_t n = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; n < ng.size;
+         n += (uint64_t)gridDim.x * blockDim.x)
+        nw[n] = old[regrid_source(og, ng, n)];
+}
+
+// savePlaneXZ<float, 4> (ComputeUnitOutput.hpp:354-447) for every plane cell
+// (i-major); flags bit 0: a QVec record (cell not excluded), bit 1: an F3 record
+// (Force::isNotZero: all three components non-zero, QVec.hpp:54-56).
+template <typename T, int L>
+__global__ void k_plane_xz(const T* __restrict__ Q, const T* __restrict__ F, const uint64_t* __restrict__ fmask,
+                           const uint8_t* __restrict__ exclude, Grid g, int cut, int64_t i0, int64_t k0,
+                           PlaneRec4* __restrict__ q, PlaneRec3* __restrict__ f, uint8_t* __restrict__ flags) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= (int64_t)g.xg1 * g.zg1) return;
+    const int i = (int)(t / g.zg1) + 1, k = (int)(t % g.zg1) + 1;
+    const uint32_t n = cell_index(g, i, cut, k);
+    uint8_t fl = 0;
+    if (!(exclude && exclude[n])) {
+        fl = 1;
+        PlaneRec4 r;
+        r.i = (uint16_t)(i0 + i - 1);
+        r.j = (uint16_t)(cut - 1);
+        r.k = (uint16_t)(k0 + k - 1);
+#pragma unroll
+        for (int l = 0; l < 4; l++) r.q[l] = (float)Q[qaddr<L>(g, n, l)];
+        q[t] = r;
+        if (mask_test(fmask, g, i, cut, k)) {
+            const T fx = F[3 * (uint64_t)n], fy = F[3 * (uint64_t)n + 1], fz = F[3 * (uint64_t)n + 2];
+            if (fx != T(0) && fy != T(0) && fz != T(0)) {
+                fl |= 2;
+                PlaneRec3 s;
+                s.i = (uint16_t)(i0 + i - 1);
+                s.j = (uint16_t)cut;
+                s.k = (uint16_t)(k0 + k - 1);
+                s.q[0] = (float)fx;
+                s.q[1] = (float)fy;
+                s.q[2] = (float)fz;
+                f[t] = s;
+            }
+        }
+    }
+    flags[t] = fl;
+}
+
+// calcVorticityXZ / XY (ComputeUnitOutput.hpp:113-160, 198-245) before the
+// picture: log|curl u|^2 from central differences of QVec::velocity() (j / rho,
+// no force term) read through the streaming accessor; excluded cells NaN.  The
+// twisted read of a far-side ghost cell leaves the array in the reference, so
+// such neighbours give NaN here.  axis 0: plane j = cut, (x, z); 1: k = cut, (x, y).
+template <typename PR, int L, int STREAM, int MOMSRC, int EVEN>
+__global__ void k_vorticity(const typename PR::S* __restrict__ Q, Grid g, int axis, int cut,
+                            const uint8_t* __restrict__ exclude, typename PR::T* __restrict__ out, float rho0) {
+    using T = typename PR::T;
+    const int nb = axis == 0 ? g.zg1 : g.yg1;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= (int64_t)g.xg1 * nb) return;
+    const int i = (int)(t / nb) + 1, c = (int)(t % nb) + 1;
+    const int j = axis == 0 ? cut : c, k = axis == 0 ? c : cut;
+    if (exclude && exclude[cell_index(g, i, j, k)]) {
+        out[t] = (T)NAN;
+        return;
+    }
+    auto vel = [&](int a, int b, int e, T& ux, T& uy, T& uz) {
+        if (STREAM == StreamEsotwist && (a == g.xg1 + 1 || b == g.yg1 + 1 || e == g.zg1 + 1)) {
+            ux = uy = uz = (T)NAN;
+            return;
+        }
+        T rho, jx, jy, jz;
+        cell_rho_j<PR, L, STREAM, MOMSRC, EVEN>(Q, g, a, b, e, rho, jx, jy, jz, rho0);
+        ux = jx / rho;
+        uy = jy / rho;
+        uz = jz / rho;
+    };
+    T x5, y5, z5, x6, y6, z6, x3, y3, z3, x4, y4, z4, x1, y1, z1, x2, y2, z2;
+    vel(i, j, k + 1, x5, y5, z5);
+    vel(i, j, k - 1, x6, y6, z6);
+    const T uxy = T(0.5) * (x5 - x6);
+    vel(i, j + 1, k, x3, y3, z3);
+    vel(i, j - 1, k, x4, y4, z4);
+    const T uxz = T(0.5) * (x3 - x4);
+    vel(i + 1, j, k, x1, y1, z1);
+    vel(i - 1, j, k, x2, y2, z2);
+    const T uyx = T(0.5) * (y1 - y2);
+    const T uyz = T(0.5) * (y3 - y4);
+    const T uzx = T(0.5) * (z1 - z2);
+    const T uzy = T(0.5) * (z5 - z6);
+    const T uxyuyx = uxy - uyx;
+    const T uyzuzy = uyz - uzy;
+    const T uzxuxz = uzx - uxz;
+    out[t] = (T)log((T)(uyzuzy * uyzuzy + uzxuxz * uzxuxz + uxyuyx * uxyuyx));
+}
+
 // ------------------------------------------------------------------ geometry on the device
 template <typename T>
 __device__ __forceinline__ void blade_gen_point(const BladeLayout& bl, const BladeStep& st, const PointsDev& out,
@@ -1374,6 +1487,63 @@ hipError_t launch_macroscopic(int prec, int layout, int streaming, int momsrc, i
             hipLaunchKernelGGL((k_macroscopic<PR, decltype(Lc)::value, decltype(Sc)::value, decltype(Mc)::value,
                                               decltype(Ec)::value>),
                                dim3(nb), dim3(kBlock), 0, s, (const S*)Q, (const T*)F, fmask, g, rho0, (T*)out);
+            return hipGetLastError();
+        });
+    });
+    return e;
+}
+
+hipError_t launch_regrid(int esize, int layout, const void* oldQ, void* newQ, const void* oldNu, void* newNu,
+                         const Grid& og, const Grid& ng, hipStream_t s) {
+    const uint32_t nb = grid_blocks(ng.size);
+    if (esize == 4) {
+        if (layout == LayoutIJKL)
+            hipLaunchKernelGGL((k_regrid_q<uint32_t, LayoutIJKL>), dim3(nb), dim3(kBlock), 0, s, (const uint32_t*)oldQ, (uint32_t*)newQ, og, ng);
+        else
+            hipLaunchKernelGGL((k_regrid_q<uint32_t, LayoutLIJK>), dim3(nb), dim3(kBlock), 0, s, (const uint32_t*)oldQ, (uint32_t*)newQ, og, ng);
+        hipLaunchKernelGGL(k_regrid_cells<uint32_t>, dim3(nb), dim3(kBlock), 0, s, (const uint32_t*)oldNu, (uint32_t*)newNu, og, ng);
+    } else if (esize == 8) {
+        if (layout == LayoutIJKL)
+            hipLaunchKernelGGL((k_regrid_q<uint64_t, LayoutIJKL>), dim3(nb), dim3(kBlock), 0, s, (const uint64_t*)oldQ, (uint64_t*)newQ, og, ng);
+        else
+            hipLaunchKernelGGL((k_regrid_q<uint64_t, LayoutLIJK>), dim3(nb), dim3(kBlock), 0, s, (const uint64_t*)oldQ, (uint64_t*)newQ, og, ng);
+        hipLaunchKernelGGL(k_regrid_cells<uint64_t>, dim3(nb), dim3(kBlock), 0, s, (const uint64_t*)oldNu, (uint64_t*)newNu, og, ng);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_plane_xz(int prec, int layout, const void* Q, const void* F, const uint64_t* fmask,
+                           const uint8_t* exclude, const Grid& g, int cut, int64_t i0, int64_t k0, PlaneRec4* q,
+                           PlaneRec3* f, uint8_t* flags, hipStream_t s) {
+    const uint32_t nb = (uint32_t)(((int64_t)g.xg1 * g.zg1 + kBlock - 1) / kBlock);
+#define TDLB_PLANE(T, L)                                                                                       \
+    hipLaunchKernelGGL((k_plane_xz<T, L>), dim3(nb), dim3(kBlock), 0, s, (const T*)Q, (const T*)F, fmask, exclude, \
+                       g, cut, i0, k0, q, f, flags)
+    if (prec == 1) {
+        if (layout == LayoutIJKL) TDLB_PLANE(double, LayoutIJKL);
+        else TDLB_PLANE(double, LayoutLIJK);
+    } else {
+        if (layout == LayoutIJKL) TDLB_PLANE(float, LayoutIJKL);
+        else TDLB_PLANE(float, LayoutLIJK);
+    }
+#undef TDLB_PLANE
+    return hipGetLastError();
+}
+
+hipError_t launch_vorticity(int prec, int layout, int streaming, int momsrc, int even, const void* Q, const Grid& g,
+                            int axis, int cut, const uint8_t* exclude, void* out, float rho0, hipStream_t s) {
+    const int64_t cells = (int64_t)g.xg1 * (axis == 0 ? g.zg1 : g.yg1);
+    const uint32_t nb = (uint32_t)((cells + kBlock - 1) / kBlock);
+    hipError_t e = hipErrorInvalidValue;
+    TDLB_WITH_PREC(prec, {
+        using S = typename PR::S;
+        using T = typename PR::T;
+        e = with_cell_source<PR>(layout, streaming, momsrc, even, [&](auto Lc, auto Sc, auto Mc, auto Ec) {
+            hipLaunchKernelGGL((k_vorticity<PR, decltype(Lc)::value, decltype(Sc)::value, decltype(Mc)::value,
+                                            decltype(Ec)::value>),
+                               dim3(nb), dim3(kBlock), 0, s, (const S*)Q, g, axis, cut, exclude, (T*)out, rho0);
             return hipGetLastError();
         });
     });

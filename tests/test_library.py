@@ -1,7 +1,7 @@
 """CPU-side checks of the C-ABI library: it loads, exports every declared entry
 point, and its host geometry generator satisfies the invariants that pin it
-(the reference's generator library is absent, so its lists are this project's
-definition)."""
+(the reference's generator library is absent, so its lists are this library's
+own definition, documented in tdlbcpp_amd/csrc/rushton.h)."""
 import math
 import os
 import subprocess

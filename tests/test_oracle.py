@@ -407,3 +407,80 @@ def test_esotwist_conserves_mass_without_forcing():
     r = cu.read_cells()
     assert r[..., 0].sum() == pytest.approx(mass0, rel=1e-13)
     np.testing.assert_allclose(r[..., 1:4].sum(axis=(0, 1, 2)), mom0, atol=1e-12)
+
+
+# ---------------------------------------------------------------- regrid, output planes, vorticity
+@pytest.mark.parametrize("layout", [O.IJKL, O.LIJK], ids=["IJKL", "LIJK"])
+def test_double_resolution_replicates_each_cell_into_a_2x2x2_block(layout):
+    """doubleResolutionFullCU (ComputeUnit.hpp:229-316): the doubled grid is the
+    old one upsampled by replication -- interior cell a of the old grid fills
+    cells 2a-1 and 2a on every axis, the ghost layers stay ghost layers."""
+    o = O.OracleCU(3, 4, 2, dtype=O.TAGS, layout=layout, streaming=O.SIMPLE)
+    q, f, nu = O.fill_for_test(o)
+    o.set_q_ijkl(q)
+    o.set(O.FIELD_NU, nu)
+    o.set(O.FIELD_F, f)
+    old_q = o.q_ijkl().copy()
+    old_nu = o.get(O.FIELD_NU).reshape(o.xg, o.yg, o.zg).copy()
+    o.double_resolution()
+    assert (o.xg, o.yg, o.zg) == (8, 10, 6)
+    new_q = o.q_ijkl()
+    new_nu = o.get(O.FIELD_NU).reshape(o.xg, o.yg, o.zg)
+
+    def src(a, n_old_interior):  # new index -> old index along one axis
+        if a == 0:
+            return 0
+        if a == 2 * n_old_interior + 1:
+            return n_old_interior + 1
+        return (a + 1) // 2
+
+    for i in range(o.xg):
+        for j in range(o.yg):
+            for k in range(o.zg):
+                s = (src(i, 3), src(j, 4), src(k, 2))
+                np.testing.assert_array_equal(new_q[i, j, k], old_q[s])
+                assert new_nu[i, j, k] == old_nu[s]
+    assert not o.get(O.FIELD_F).any() and not o.get(O.FIELD_O).any()
+
+
+def test_plane_xz_records_hold_raw_slots_and_nonzero_forces():
+    """savePlaneXZ (ComputeUnitOutput.hpp:354-447): one record per plane cell not
+    excluded, the first 4 raw slots of the cell; an F record where all three
+    components of F are non-zero (Force::isNotZero, QVec.hpp:54-56)."""
+    o = O.OracleCU(4, 3, 5, dtype=O.F32, layout=O.LIJK, streaming=O.ESOTWIST)
+    rng = np.random.default_rng(3)
+    q4 = rng.standard_normal((o.xg, o.yg, o.zg, 18)).astype(np.float32)
+    o.set_q_ijkl(q4)
+    F = np.zeros((o.xg, o.yg, o.zg, 3), np.float32)
+    F[2, 2, 3] = (1.0, -2.0, 0.5)
+    F[1, 2, 1] = (0.0, 0.0, 3.0)
+    o.set(O.FIELD_F, F.ravel())
+    ex = np.zeros((o.xg, o.yg, o.zg), np.uint8)
+    ex[3, 2, 2] = 1
+    qr, fr = o.plane_xz(2, ex.ravel(), i0=10, k0=20)
+    want = [(i, k) for i in range(1, 5) for k in range(1, 6) if not ex[i, 2, k]]
+    assert len(qr) == len(want) == 19
+    for r, (i, k) in zip(qr, want):
+        assert (r["i"], r["j"], r["k"]) == (10 + i - 1, 1, 20 + k - 1)
+        np.testing.assert_array_equal(r["q"], q4[i, 2, k, :4])
+    assert [(r["i"], r["j"], r["k"]) for r in fr] == [(11, 2, 22)]  # (0, 0, 3) is "zero"
+    np.testing.assert_array_equal(fr[0]["q"], F[2, 2, 3])
+
+
+def test_vorticity_of_a_shear_flow_is_log_of_shear_squared():
+    """calcVorticityXZ (ComputeUnitOutput.hpp:113-160) on u_x = a k: the only
+    velocity gradient is du_x/dk = a, so the field is log(a^2) in the interior."""
+    o = O.OracleCU(5, 5, 7, dtype=O.F64, layout=O.IJKL, streaming=O.SIMPLE)
+    a = 0.01
+    m = np.zeros(o.interior_shape())
+    m[..., 0] = 1.0
+    m[..., 1] = a * np.arange(1, 8)[None, None, :]
+    o.write_cells(m, as_moments=True)
+    v = o.vorticity(0, 3)
+    # cells next to the ghost layer read zero-density ghosts (0/0): interior of the plane only
+    np.testing.assert_allclose(v[1:-1, 1:-1], np.log(a * a), rtol=1e-12)
+    ex = np.zeros(o.size, np.uint8)
+    ex[o.q_ijkl().shape[1] * o.zg * 2 + o.zg * 3 + 4] = 1  # cell (2, 3, 4)
+    assert np.isnan(o.vorticity(0, 3, ex)[1, 3])
+    # plane k = 4: same shear, same value away from the ghost layers
+    np.testing.assert_allclose(o.vorticity(1, 4)[1:-1, 1:-1], np.log(a * a), rtol=1e-12)
