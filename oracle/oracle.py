@@ -69,6 +69,7 @@ def lib():
         L.orc_moments_of_f64.argtypes = [vp, vp]
         L.orc_moments_of_f32.argtypes = [vp, vp]
         L.orc_double_resolution.argtypes = [vp]
+        L.orc_set_flow.argtypes = [vp, ctypes.POINTER(d), ctypes.c_int]
         L.orc_plane_xz.argtypes = [vp, i64, vp, i64, i64, vp, ctypes.POINTER(i64), vp, ctypes.POINTER(i64)]
         L.orc_vorticity.argtypes = [vp, ctypes.c_int, i64, vp, vp]
         _lib = L
@@ -98,6 +99,14 @@ class OracleCU:
         self._h = lib().orc_create(dtype, layout, streaming, x, y, z, ghost, arr, 1 if use_les else 0)
         self.xg, self.yg, self.zg, self.size = (lib().orc_dim(self._h, w) for w in range(4))
         self.ghost = ghost
+        self.flow = vals
+        self.use_les = use_les
+
+    def set_flow(self, **flow):
+        """Replace flow parameters (the ComputeUnit's `flow` member)."""
+        self.flow.update(flow)
+        arr = (ctypes.c_double * 14)(*[float(self.flow[k]) for k in FLOW_KEYS])
+        lib().orc_set_flow(self._h, arr, 1 if self.use_les else 0)
 
     def __del__(self):
         if getattr(self, "_h", None):

@@ -80,7 +80,11 @@ struct Base {
                          int64_t* nf) = 0;
     virtual void vorticityXZ(int64_t cut, const uint8_t* exclude, void* out) = 0;
     virtual void vorticityXY(int64_t cut, const uint8_t* exclude, void* out) = 0;
+    virtual void setFlow(const double* v, int useLES) = 0;
 };
+
+template <typename T>
+Flow<T> makeFlow(const double* v, int useLES);
 
 template <typename T, int LAYOUT, int STREAM>
 struct CU : Base {
@@ -106,6 +110,8 @@ struct CU : Base {
     }
 
     bool& even() override { return evenStep; }
+    // the ComputeUnit's `flow` member replaced (e.g. nu after calcNuAndRe_m)
+    void setFlow(const double* v, int useLES) override { flow = makeFlow<T>(v, useLES); }
     int64_t dims(int w) override {
         const int64_t v[] = {xg, yg, zg, (int64_t)size};
         return v[w];
@@ -675,6 +681,7 @@ int orc_even_step(void* h) { return static_cast<orc::Base*>(h)->even() ? 1 : 0; 
 void orc_set_even_step(void* h, int e) { static_cast<orc::Base*>(h)->even() = (e != 0); }
 int64_t orc_dim(void* h, int which) { return static_cast<orc::Base*>(h)->dims(which); }
 void orc_double_resolution(void* h) { static_cast<orc::Base*>(h)->doubleResolution(); }
+void orc_set_flow(void* h, const double* flow, int useLES) { static_cast<orc::Base*>(h)->setFlow(flow, useLES); }
 // records of 24 bytes {u16 i, j, k; f32 q[4]} and 20 bytes {u16 i, j, k; f32 q[3]}; x*z capacity each
 void orc_plane_xz(void* h, int64_t cut, const uint8_t* exclude, int64_t i0, int64_t k0, void* q, int64_t* nq,
                   void* f, int64_t* nf) {

@@ -143,6 +143,16 @@ public:
         tdlb_flow cf = to_c(f);
         throw_if(tdlb_cu_set_flow(h_, &cf), "setFlow");
     }
+    // doubleResolutionFullCU (ComputeUnit.hpp:257-316): sizes double, resolution += 2
+    void doubleResolutionFullCU() {
+        throw_if(tdlb_cu_double_resolution(h_), "doubleResolutionFullCU");
+        x *= 2; y *= 2; z *= 2;
+        resolution += 2;
+        xg = x + 2 * ghost; yg = y + 2 * ghost; zg = z + 2 * ghost;
+        xg0 = xg - 1; yg0 = yg - 1; zg0 = zg - 1;
+        xg1 = xg - 2; yg1 = yg - 2; zg1 = zg - 2;
+        size = (size_t)xg * yg * zg;
+    }
 
     // ---- checkpoint (ComputeUnit.hpp:322-439), JSON and raw arrays byte-compatible.
     // The AllParams JSON carries one extension key, "ComputeUnitState": {"evenStep"},

@@ -132,12 +132,6 @@ int main(int argc, char** argv) {
         std::cout << j.dump(4) << std::endl;
         return 0;
     }
-    if (running.doubleResolutionAtStep > running.step && running.doubleResolutionAtStep <= running.num_steps + 1) {
-        std::cerr << "doubleResolutionAtStep " << running.doubleResolutionAtStep
-                  << " falls inside the run: regridding (doubleResolutionFullCU) is not supported; "
-                     "set it beyond num_steps" << std::endl;
-        return EXIT_FAILURE;
-    }
 
     try {
         DiskOutputTree tree(output, checkpoint);
@@ -175,6 +169,41 @@ int main(int argc, char** argv) {
             lb.forcing(all, flow.alpha, flow.beta);
         }
         for (tStep step = first; step <= running.num_steps; step++) {
+            if (running.step + 1 == running.doubleResolutionAtStep) {
+                // main.cpp:349-399, at the top of the step that doubles: regrid, the
+                // tank of the new size, angle += its increment, forcing on the new lists
+                lb.doubleResolutionFullCU();
+                cu.x = lb.x; cu.y = lb.y; cu.z = lb.z; cu.resolution = lb.resolution;
+                grid.x = lb.x; grid.y = lb.y; grid.z = lb.z;
+                throw_if(tdlb_rushton_defaults(lb.x, lb.y, flow.uav, (int64_t)running.impellerStartupStepsUntilNormalSpeed,
+                                               &rp),
+                         "rushton_defaults");
+                if (!nu_from_json) {
+                    tdlb_flow cf = ComputeUnit::to_c(flow);
+                    throw_if(tdlb_flow_calc_nu_re_m(&cf, (int32_t)rp.blade_outer_radius, precision), "calcNuAndRe_m");
+                    flow.nu = cf.nu;
+                    flow.Re_m = cf.re_m;
+                    lb.setFlow(flow);
+                }
+                tree.setParams(cu, grid, flow.getJson(), running, output, checkpoint);
+                const tStep sd = running.step + 1;
+                const double d0 = inc(sd);
+                running.angle += d0;
+                if (!per_call) {
+                    throw_if(tdlb_cu_bind_rushton(lb.handle(), &rp, sd, running.angle, flow.alpha, flow.beta),
+                             "bind_rushton");
+                } else {
+                    fixed = to_pos(generate(rp, TDLB_GEOM_FIXED, 0, 0));
+                    nonupd = to_pos(generate(rp, TDLB_GEOM_ROTATING_NON_UPDATING, 0, d0));
+                    std::vector<PosPolar> all = fixed;
+                    all.insert(all.end(), nonupd.begin(), nonupd.end());
+                    auto rot = to_pos(generate(rp, TDLB_GEOM_ROTATING, running.angle, d0));
+                    all.insert(all.end(), rot.begin(), rot.end());
+                    lb.forcing(all, flow.alpha, flow.beta);
+                }
+                std::printf("INCREASING RESOLUTION TO grid.x %lld at step %lu\n", (long long)lb.x,
+                            (unsigned long)sd);
+            }
             if (!per_call) {
                 uint64_t s = running.step;
                 double a = running.angle;

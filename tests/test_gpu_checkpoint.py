@@ -117,3 +117,47 @@ def test_driver_config1_matches_oracle(tmp_path, mode):
     assert_close_moments(n, o.get(O.FIELD_NU), what="Nu", rho0=1e-3)
     print(f"[gpu case] config 1 bit-identical Q: {np.array_equal(q, o.get(O.FIELD_Q))}", flush=True)
     assert len(glob.glob(str(tmp_path / "cfg1_checkpoint" / "*"))) == 2
+
+
+@pytest.mark.parametrize("mode", ["fused", "per-call"])
+def test_driver_regrid_inside_the_run_matches_oracle(tmp_path, mode):
+    """doubleResolutionAtStep inside the run (main.cpp:349-399): the grid doubles
+    at the top of that step, the tank of the new size is bound with angle +=
+    its increment, nu is re-derived for the new blade radius, and the run goes
+    on; the checkpoint at the last step equals the oracle's."""
+    case(f"driver regrid {mode}")
+    inp, j = _config1_input(tmp_path)
+    T, steps, at = 36, 8, 5
+    j["GridParams"].update(x=T, y=T, z=T)
+    j["RunningParams"].update(num_steps=steps, doubleResolutionAtStep=at)
+    j["CheckpointParams"].update(checkpointRepeat=steps)
+    inp.write_text(json.dumps(j))
+    args = [os.path.join(ROOT, "tdlbcpp_amd", "tdlb_run"), "-i", str(inp)]
+    if mode == "per-call":
+        args.append("--per-call")
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "INCREASING RESOLUTION TO grid.x 72" in r.stdout
+    d = tmp_path / "cfg1_checkpoint" / f"cfg1_checkpoint_step_{steps}"
+    allp = json.loads((d / "AllParams.0.0.0.device.json").read_text())
+    assert allp["ComputeUnitParams"]["x"] == 2 * T and allp["ComputeUnitParams"]["resolution"] == 3
+    flow = dict(initialRho=1.0, cs0=0.1, g3=0.8, alpha=0.97, beta=1.9)
+    nu, _ = O.calc_nu(7300.0, 0.1, int(T / 6), O.F32)
+    o = O.OracleCU(T, dtype=O.F32, layout=O.IJKL, streaming=O.SIMPLE, use_les=True, nu=nu, **flow)
+    o.initialise(1.0)
+    rp = td.rushton_defaults(T, tip_speed=0.1, startup_steps=30)
+    ref = ReferenceLoop(o, rp)
+    ref.run(at - 2)  # steps 2 .. at-1
+    o.double_resolution()
+    rp2 = td.rushton_defaults(2 * T, tip_speed=0.1, startup_steps=30)
+    nu2, _ = O.calc_nu(7300.0, 0.1, int(2 * T / 6), O.F32)
+    o.set_flow(nu=nu2)
+    angle = ref.angle + ref.rnd(td.rushton_step_increment(rp2, at))
+    ref2 = ReferenceLoop(o, rp2, step=at, angle=angle)
+    ref2.step = at - 1  # the regular body of step `at` follows
+    ref2.run(steps - at + 1)
+    q = np.fromfile(d / "checkpoint_grid.0.0.0.device.Q", dtype=np.float32)
+    f = np.fromfile(d / "checkpoint_grid.0.0.0.device.F", dtype=np.float32)
+    assert_close_moments(q, o.get(O.FIELD_Q), what="Q")
+    assert_close_moments(f, o.get(O.FIELD_F), what="F", rho0=1e-3)
+    assert np.array_equal(q, o.get(O.FIELD_Q))
