@@ -131,13 +131,15 @@ hipError_t launch_macroscopic(int prec, int layout, int streaming, int momsrc, i
 hipError_t launch_regrid(int esize, int layout, const void* oldQ, void* newQ, const void* oldNu, void* newNu,
                          const Grid& og, const Grid& ng, hipStream_t s);
 
-// records of savePlaneXZ<float, 4> / <float, 3> (tDiskGrid, Output.hpp:22-25)
+// records of savePlaneXZ<float, 4> / <float, 3> (tDiskGrid, Output.hpp:22-25:
+// three uint16 then the floats at offset 8); the two alignment bytes are
+// written as zero here (the reference leaves them unset)
 struct PlaneRec4 {
-    uint16_t i, j, k;
+    uint16_t i, j, k, pad;
     float q[4];
 };
 struct PlaneRec3 {
-    uint16_t i, j, k;
+    uint16_t i, j, k, pad;
     float q[3];
 };
 // one record slot and one flag byte per plane cell (x * z, i-major); prec 0/1 (explicit Q type)

@@ -952,6 +952,7 @@ __global__ void k_plane_xz(const T* __restrict__ Q, const T* __restrict__ F, con
         r.i = (uint16_t)(i0 + i - 1);
         r.j = (uint16_t)(cut - 1);
         r.k = (uint16_t)(k0 + k - 1);
+        r.pad = 0;
 #pragma unroll
         for (int l = 0; l < 4; l++) r.q[l] = (float)Q[qaddr<L>(g, n, l)];
         q[t] = r;
@@ -963,6 +964,7 @@ __global__ void k_plane_xz(const T* __restrict__ Q, const T* __restrict__ F, con
                 s.i = (uint16_t)(i0 + i - 1);
                 s.j = (uint16_t)cut;
                 s.k = (uint16_t)(k0 + k - 1);
+                s.pad = 0;
                 s.q[0] = (float)fx;
                 s.q[1] = (float)fy;
                 s.q[2] = (float)fz;
