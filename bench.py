@@ -272,9 +272,11 @@ def main():
     ap.add_argument("--variants", default="all", help="all, precision (fp32 + fp16/bf16 Esotwist/LIJK) or headline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--workload", default="tank", choices=["tank", "batch", "serve"],
+    ap.add_argument("--workload", default="tank", choices=["tank", "batch", "serve", "accuracy"],
                     help="tank: one 512^3 tank (headline); batch: independent tanks batched vs one at a time; "
-                         "serve: a Poisson stream of tank requests")
+                         "serve: a Poisson stream of tank requests; accuracy: fp32/fp16/bf16 against the fp64 "
+                         "build (use --grid 96 --steps 5000)")
+    ap.add_argument("--accuracy-re", type=float, nargs="+", default=[1.0e5, 7300.0])
     ap.add_argument("--duration", type=float, default=600.0, help="serve: seconds of arrivals")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: arrivals per second")
     ap.add_argument("--tick-steps", type=int, default=50, help="serve: steps per scheduling tick")
@@ -308,6 +310,21 @@ def main():
                "data": "synthetic: Rushton tanks at rest, geometry from this library's generator",
                "config": {"workload": f"batched_rushton_tanks_{args.tanks}_fp16_esotwist_lijk", **r}}
         print(json.dumps(out), flush=True)
+        return 0
+    if args.workload == "accuracy":
+        from tdlbcpp_amd import precision as P
+        reports = {}
+        for name, st in (("Esotwist", td.STREAM_ESOTWIST), ("Simple", td.STREAM_SIMPLE)):
+            for re_ in args.accuracy_re:
+                r = P.precision_report(T=args.grid, steps=args.steps, precisions=(td.PREC_F32, td.PREC_F16,
+                                                                                 td.PREC_BF16),
+                                       streaming=st, re=re_, samples=20)
+                reports[f"{name}/LIJK/Re{re_:g}"] = r
+                print(f"[bench] accuracy {name} Re={re_:g}: " + ", ".join(
+                    f"{k} L2 {v['velocity_l2']:.2e} Po err {v['po_rel_err']:.2e}" for k, v in r["variants"].items()),
+                    file=sys.stderr, flush=True)
+        print(json.dumps({"workload": f"accuracy_{args.grid}cubed_{args.steps}steps_vs_fp64", "reports": reports}),
+              flush=True)
         return 0
     if args.workload == "serve":
         r = run_serve_workload(td, args.duration, args.rate, args.tick_steps, args.max_resident)

@@ -1,5 +1,5 @@
 """Shared helpers of the parity tests: the reference main-loop sequence on the
-oracle, seeded states and the tolerance of the north-star parity bar."""
+oracle, seeded states and the tolerance of the parity bar."""
 import numpy as np
 
 import tdlbcpp_amd as td
