@@ -263,6 +263,28 @@ def copy_calibration(gib):
     return 0
 
 
+def measured_copy_gbs(gib=2.0, reps=20):
+    """Bytes read + written per second of torch's device copy of `gib` GiB,
+    timed with events on torch's stream: the bandwidth a plain streaming
+    kernel reaches on this GPU, beside the 8 TB/s peak."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    gbs = 2 * reps * n * 4 / (e0.elapsed_time(e1) / 1e3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -379,6 +401,9 @@ def main():
                      "kernel": "k_esotwist<float,LIJK,FUSED>",
                      "bytes_per_cell": bytes_per_cell(), "cells_per_launch": args.grid ** 3},
     }
+    copy = measured_copy_gbs()
+    out["roofline"]["copy_gbs"] = round(copy, 1)
+    out["roofline"]["frac_of_copy"] = round(achieved / copy, 4)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(max_seconds=args.cpu_seconds)
     print(json.dumps(out), flush=True)

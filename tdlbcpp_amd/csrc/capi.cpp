@@ -50,6 +50,7 @@ int fail(int code, const std::string& msg) {
 
 namespace {
 
+
 int set_device(tdlb_cu* cu) {
     HIPCHK(hipSetDevice(cu->device));
     return TDLB_OK;
@@ -413,6 +414,11 @@ int tdlb_cu_create(const tdlb_cu_params* p, const tdlb_flow* f, tdlb_cu_t* out) 
     cu->g.size = size;
     cu->tsize = p->precision == TDLB_PREC_F64 ? 8 : 4;
     cu->ssize = p->precision == TDLB_PREC_F64 ? 8 : p->precision == TDLB_PREC_F32 ? 4 : 2;
+    set_q_geometry(cu->g, cu->dlay(), cu->ssize);
+    if (cu->q_cells() >= (1ull << 32)) {
+        delete cu;
+        return fail(TDLB_ERR_UNSUPPORTED, "grid too large for 32-bit cell indexing");
+    }
     cu->rho0 = (float)f->initial_rho;
     cu->mask_words = (size_t)xg * yg * cu->g.mask_w;
     cu->device = p->device_id < 0 ? 0 : p->device_id;
@@ -607,7 +613,7 @@ int tdlb_cu_forcing(tdlb_cu_t cu, const tdlb_ib_point* points, int64_t n, double
 int tdlb_cu_field_bytes(tdlb_cu_t cu, int field, size_t* bytes) {
     if (!cu || !bytes) return fail(TDLB_ERR_INVALID, "null argument");
     switch (field) {
-        case TDLB_FIELD_Q: *bytes = cu->q_bytes(); break;
+        case TDLB_FIELD_Q: *bytes = cu->qb_bytes(); break;
         case TDLB_FIELD_F: *bytes = cu->f_bytes(); break;
         case TDLB_FIELD_NU: *bytes = cu->nu_bytes(); break;
         case TDLB_FIELD_O: *bytes = cu->g.size; break;
@@ -632,10 +638,10 @@ int tdlb_cu_download(tdlb_cu_t cu, int field, void* host, size_t bytes) {
         if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("download F: ") + hipGetErrorString(e));
         return TDLB_OK;
     }
-    if (field == TDLB_FIELD_Q && cu->blay() != cu->dlay()) {
+    if (field == TDLB_FIELD_Q && cu->q_converted()) {
         void* tmp = nullptr;
         if (hipMalloc(&tmp, need) != hipSuccess) return fail(TDLB_ERR_NOMEM, "download Q staging");
-        hipError_t e = launch_relayout_q((int)cu->tsize, cu->blay(), cu->eq(), tmp, cu->g, cu->stream);
+        hipError_t e = launch_q_boundary((int)cu->tsize, cu->dlay(), cu->blay(), 1, cu->eq(), tmp, cu->g, cu->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(host, tmp, need, hipMemcpyDeviceToHost, cu->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(cu->stream);
         (void)hipFree(tmp);
@@ -656,11 +662,11 @@ int tdlb_cu_upload(tdlb_cu_t cu, int field, const void* host, size_t bytes) {
     CHK(materialise(cu));
     void* dst = field == TDLB_FIELD_Q ? cu->eq() : field == TDLB_FIELD_F ? cu->F
               : field == TDLB_FIELD_NU ? cu->Nu : (void*)cu->O;
-    if (field == TDLB_FIELD_Q && cu->blay() != cu->dlay()) {
+    if (field == TDLB_FIELD_Q && cu->q_converted()) {
         void* tmp = nullptr;
         if (hipMalloc(&tmp, need) != hipSuccess) return fail(TDLB_ERR_NOMEM, "upload Q staging");
         hipError_t e = hipMemcpyAsync(tmp, host, need, hipMemcpyHostToDevice, cu->stream);
-        if (e == hipSuccess) e = launch_relayout_q((int)cu->tsize, cu->dlay(), tmp, dst, cu->g, cu->stream);
+        if (e == hipSuccess) e = launch_q_boundary((int)cu->tsize, cu->dlay(), cu->blay(), 0, tmp, dst, cu->g, cu->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(cu->stream);
         (void)hipFree(tmp);
         if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("upload Q: ") + hipGetErrorString(e));

@@ -1,4 +1,4 @@
-// State of a ComputeUnit handle (private to the library: capi.cpp, batch.cpp).
+// State of a ComputeUnit handle (private to the library: capi.cpp, batch.cpp, server.cpp).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -59,6 +59,34 @@ struct DevPoints {
         return TDLB_OK;
     }
 };
+
+// Population arrays in LIJK: a k-row holds the interior cells k = 1..zg1 only,
+// rows back to back (ghost rows in i and j included), and the two k-ghost
+// cells of every row sit in a side region after the rows.  The stream-collide
+// kernels never touch a k-ghost (Esotwist wraps periodically, the Simple pull
+// reads interior neighbours only), so every line they move holds interior
+// cells only, and at zg1 = 512 every row starts a 128 B line.  Slot planes of
+// 16 MiB and more start on 1 MiB boundaries (planes of 18 concurrent streams
+// whose starts fall at arbitrary offsets measured 4 % slower at 512^3), smaller
+// ones on 128 B lines.  IJKL keeps the dense index.
+inline void set_q_geometry(Grid& g, int dlay, size_t ssize) {
+    g.qside = 0;
+    if (dlay == LayoutIJKL) {
+        g.qz = (uint32_t)g.zg;
+        g.qyz = g.yz;
+        g.qoff = 0;
+        g.qsize = g.size;
+        return;
+    }
+    g.qz = (uint32_t)g.zg1;
+    g.qyz = (uint32_t)((uint64_t)g.yg * g.qz);
+    g.qoff = 0xffffffffu;  // (i, j, k) at i*qyz + j*qz + k - 1 (modulo 2^32)
+    const uint64_t line = 128 / ssize;  // cells of the stored type in a 128 B line
+    g.qside = (uint32_t)(((uint64_t)g.xg * g.qyz + line - 1) / line * line);
+    const uint64_t cells = (uint64_t)g.qside + 2ull * g.xg * g.yg;
+    const uint64_t align = cells * ssize >= (16ull << 20) ? (1ull << 20) / ssize : line;
+    g.qsize = (cells + align - 1) / align * align;
+}
 
 // Stencil cells of a point list with a moments cache: the sorted unique cells,
 // and for every point the 9 positions of its stencil cells in that list.
@@ -151,8 +179,16 @@ struct tdlb_cu {
     std::vector<std::pair<int, int>> coll, force;
     size_t ev_used = 0;
 
-    size_t q_bytes() const { return (size_t)g.size * kQ * tsize; }   // Q as the reference holds it
-    size_t qs_bytes() const { return (size_t)g.size * kQ * ssize; }  // one population lattice
+    // cells per slot of the kernels' population arrays (rows padded in LIJK)
+    size_t q_cells() const { return dlay() == LayoutIJKL ? (size_t)g.size : (size_t)g.qsize; }
+    size_t qb_bytes() const { return (size_t)g.size * kQ * tsize; }  // Q as the reference holds it (boundary)
+    size_t q_bytes() const { return q_cells() * kQ * tsize; }        // explicit Q on the device
+    size_t qs_bytes() const { return q_cells() * kQ * ssize; }       // one population lattice
+    // Q at the boundary differs from the device array (layout or row padding)
+    bool q_converted() const {
+        return blay() != dlay() || g.qz != (uint32_t)g.zg || g.qoff != 0 || g.qside != 0 ||
+               (size_t)g.qsize != (size_t)g.size;
+    }
     size_t f_bytes() const { return (size_t)g.size * 3 * tsize; }
     size_t nu_bytes() const { return (size_t)g.size * tsize; }
     size_t mask_bytes() const { return mask_words * sizeof(uint64_t); }

@@ -77,8 +77,11 @@ hipError_t launch_push_all(int esize, int layout, const void* src, void* dst, co
 hipError_t launch_pull_all(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s);
 hipError_t launch_bounce_back(int esize, int layout, void* Q, const Grid& g, hipStream_t s);
 hipError_t launch_ghost_fill(int esize, int layout, const void* src, void* dst, const Grid& g, hipStream_t s);
-// Q (size x 18 elements of esize bytes, 4 or 8) written in layout `to` from the other layout
-hipError_t launch_relayout_q(int esize, int to, const void* src, void* dst, const Grid& g, hipStream_t s);
+// Q (elements of esize bytes, 4 or 8) between the kernels' population array
+// (layout dlayout, padded rows) and the dense size x 18 array of the boundary
+// (layout blayout): to_boundary != 0 reads the former and writes the latter.
+hipError_t launch_q_boundary(int esize, int dlayout, int blayout, int to_boundary, const void* src, void* dst,
+                             const Grid& g, hipStream_t s);
 // ghost fill from reduced-precision populations (prec 2/3) into a float array
 hipError_t launch_ghost_fill_decode(int prec, int layout, const void* src, void* dst, const Grid& g, float rho0,
                                     hipStream_t s);

@@ -25,6 +25,14 @@ namespace tdlb {
 namespace {
 
 constexpr int kBlock = 256;
+// Stream-collide kernels with 16-bit storage take one block per interior k-row
+// when the row fits (up to kRowBlock lanes), so a 128 B line of a slot plane is
+// read by one block only, except the line two neighbouring rows share (measured
+// 3-4 % faster at 512^3 than 256-lane blocks).  32- and 64-bit storage keep
+// 256-lane blocks, which measured 2-19 % faster than 512 there.
+constexpr int kRowBlock = 512;
+template <typename S>
+constexpr int row_block_max() { return sizeof(S) == 2 ? kRowBlock : kBlock; }
 
 inline uint32_t blocks_for(uint64_t n, int bs = kBlock) {
     uint64_t b = (n + bs - 1) / bs;
@@ -75,46 +83,70 @@ struct RawBits {
     static constexpr bool kShift = false;
 };
 
-template <typename PR>
+// Streaming accesses (NT): the stream-collide kernels touch every population
+// once per step; kStreamNT gives their loads and stores the non-temporal hint.
+#ifndef TDLB_STREAM_NT
+#define TDLB_STREAM_NT 0
+#endif
+constexpr bool kStreamNT = TDLB_STREAM_NT != 0;
+template <bool NT, typename W>
+__device__ __forceinline__ W ld(const W* p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, typename W>
+__device__ __forceinline__ void st(W* p, W v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <typename PR, bool NT = false>
 __device__ __forceinline__ typename PR::T load_pop(const typename PR::S* __restrict__ Q, uint64_t a, int d,
                                                    float rho0) {
-    if constexpr (PR::kShift) return ((float)Q[a] + rho0) * (1.0f / rest_weight_inv(d));
-    else return Q[a];
+    if constexpr (PR::kShift) return ((float)ld<NT>(Q + a) + rho0) * (1.0f / rest_weight_inv(d));
+    else return ld<NT>(Q + a);
 }
-template <typename PR>
+template <typename PR, bool NT = false>
 __device__ __forceinline__ void store_pop(typename PR::S* __restrict__ Q, uint64_t a, int d, typename PR::T v,
                                           float rho0) {
-    if constexpr (PR::kShift) Q[a] = (typename PR::S)(v * rest_weight_inv(d) - rho0);
-    else Q[a] = v;
+    if constexpr (PR::kShift) st<NT>(Q + a, (typename PR::S)(v * rest_weight_inv(d) - rho0));
+    else st<NT>(Q + a, v);
 }
 
 // ------------------------------------------------------------------ fills
+// every slot of the population array, row padding included
 template <typename T, int L>
 __global__ void k_fill_q(T* __restrict__ Q, Grid g, T rho) {
-    const uint64_t total = g.size * (uint64_t)kQ;
+    const uint64_t cells = L == LayoutIJKL ? g.size : g.qsize;
+    const uint64_t total = cells * (uint64_t)kQ;
     for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < total;
          a += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t slot = (L == LayoutIJKL) ? a % kQ : a / g.size;
+        const uint64_t slot = (L == LayoutIJKL) ? a % kQ : a / cells;
         Q[a] = slot == 0 ? rho : T(0);
     }
 }
 
-// one destination element per thread: IJKL slot n*18 + l <-> LIJK slot l*size + n
-template <typename W, int TO>
-__global__ void k_relayout_q(const W* __restrict__ src, W* __restrict__ dst, Grid g) {
+// Between the population array of the kernels (layout DL, padded rows) and the
+// dense array of the boundary (layout BL: IJKL n*18 + l, LIJK l*size + n); one
+// boundary element per thread.
+template <typename W, int DL, int BL, bool TO_BOUNDARY>
+__global__ void k_q_boundary(const W* __restrict__ src, W* __restrict__ dst, Grid g) {
     const uint64_t total = g.size * (uint64_t)kQ;
     for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < total;
          a += (uint64_t)gridDim.x * blockDim.x) {
         uint64_t n, l;
-        if (TO == LayoutLIJK) {
+        if (BL == LayoutLIJK) {
             l = a / g.size;
             n = a - l * g.size;
-            dst[a] = src[n * kQ + l];
         } else {
             n = a / kQ;
             l = a - n * kQ;
-            dst[a] = src[l * g.size + n];
         }
+        int i, j, k;
+        decode_cell(g, n, i, j, k);
+        const uint64_t q = qaddr<DL>(g, qcell(g, i, j, k), (int)l);
+        if (TO_BOUNDARY) dst[a] = src[q];
+        else dst[q] = src[a];
     }
 }
 
@@ -152,14 +184,15 @@ __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, ty
     using T = typename PR::T;
     using D = typename PR::D;
     const uint32_t n = cell_index(g, i, j, k);
-    const int32_t di = (i == g.xg1) ? -(int32_t)((uint32_t)(g.xg1 - 1) * g.yz) : (int32_t)g.yz;
-    const int32_t dj = (j == g.yg1) ? -(g.yg1 - 1) * g.zg : g.zg;
+    const uint32_t qn = qcell(g, i, j, k);
+    const int32_t di = (i == g.xg1) ? -(int32_t)((uint32_t)(g.xg1 - 1) * g.qyz) : (int32_t)g.qyz;
+    const int32_t dj = (j == g.yg1) ? -(int32_t)((uint32_t)(g.yg1 - 1) * g.qz) : (int32_t)g.qz;
     const int32_t dk = (k == g.zg1) ? -(g.zg1 - 1) : 1;
 
     uint32_t home[kQ];
 #pragma unroll
     for (int d = 0; d < kQ; d++)
-        home[d] = n + (uint32_t)((twist_dx(d) ? di : 0) + (twist_dy(d) ? dj : 0) + (twist_dz(d) ? dk : 0));
+        home[d] = qn + (uint32_t)((twist_dx(d) ? di : 0) + (twist_dy(d) ? dj : 0) + (twist_dz(d) ? dk : 0));
 
     T v[kQ];
 #pragma unroll
@@ -167,7 +200,7 @@ __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, ty
         const uint64_t a = qaddr<L>(g, home[d], EVEN ? d : opposite(d));
         // reduced-precision storage keeps the moments of MODE_COLL in mout
         if constexpr (PR::kShift && MODE == MODE_COLL) v[d] = mout[a];
-        else v[d] = load_pop<PR>(Q, a, d, rho0);
+        else v[d] = load_pop<PR, kStreamNT>(Q, a, d, rho0);
     }
 
     if (MODE == MODE_MOM) {
@@ -184,7 +217,7 @@ __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, ty
         fy = F[3 * (uint64_t)n + 1];
         fz = F[3 * (uint64_t)n + 2];
     }
-    T nuc = P.use_les ? Nu[n] : T(0);
+    T nuc = P.use_les ? ld<kStreamNT>(Nu + n) : T(0);
     T p[kQ];
     if (MODE == MODE_FUSED) {
         T m[kQ];
@@ -193,23 +226,23 @@ __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, ty
     } else {
         es_collide<T, D>(v, fx, fy, fz, nuc, P, p);
     }
-    if (P.use_les) Nu[n] = nuc;
+    if (P.use_les) st<kStreamNT>(Nu + n, nuc);
     // Recompute the 18 home cells instead of keeping them live through the
     // collision: the opaque copy stops the compiler from reusing the load
     // addresses, which frees ~36 VGPRs and raises occupancy.
-    uint32_t n2 = n;
+    uint32_t n2 = qn;
     int32_t di2 = di, dj2 = dj, dk2 = dk;
     asm volatile("" : "+v"(n2), "+v"(di2), "+v"(dj2), "+v"(dk2));
 #pragma unroll
     for (int d = 0; d < kQ; d++) {
         const int e = opposite(d);
         const uint32_t h = n2 + (uint32_t)((twist_dx(e) ? di2 : 0) + (twist_dy(e) ? dj2 : 0) + (twist_dz(e) ? dk2 : 0));
-        store_pop<PR>(Q, qaddr<L>(g, h, EVEN ? e : opposite(e)), d, p[d], rho0);
+        store_pop<PR, kStreamNT>(Q, qaddr<L>(g, h, EVEN ? e : opposite(e)), d, p[d], rho0);
     }
 }
 
 template <typename PR, int L, int MODE, int EVEN>
-__global__ __launch_bounds__(kBlock) void k_esotwist(typename PR::S* __restrict__ Q, typename PR::T* mout,
+__global__ __launch_bounds__(kRowBlock) void k_esotwist(typename PR::S* __restrict__ Q, typename PR::T* mout,
                                                      const typename PR::T* __restrict__ F,
                                                      typename PR::T* __restrict__ Nu,
                                                      const uint64_t* __restrict__ fmask, Grid g,
@@ -235,6 +268,7 @@ __device__ __forceinline__ void simple_cell(const typename PR::S* src,
     using T = typename PR::T;
     using D = typename PR::D;
     const uint32_t n = cell_index(g, i, j, k);
+    const uint32_t qn = qcell(g, i, j, k);
 
     T v[kQ];
     if (MODE == MODE_FUSED || MODE == MODE_MOM_PULL) {
@@ -246,14 +280,14 @@ __device__ __forceinline__ void simple_cell(const typename PR::S* src,
             const int ex = dir_x(d), ey = dir_y(d), ez = dir_z(d);
             const bool ok = (ex > 0 ? im : ex < 0 ? ip : true) && (ey > 0 ? jm : ey < 0 ? jp : true) &&
                             (ez > 0 ? km : ez < 0 ? kp : true);
-            const int32_t off = -(ex * (int32_t)g.yz + ey * g.zg + ez);
-            v[d] = load_pop<PR>(src, qaddr<L>(g, ok ? n + (uint32_t)off : n, d), d, rho0);
+            const int32_t off = -(ex * (int32_t)g.qyz + ey * (int32_t)g.qz + ez);
+            v[d] = load_pop<PR, kStreamNT>(src, qaddr<L>(g, ok ? qn + (uint32_t)off : qn, d), d, rho0);
         }
     } else {
 #pragma unroll
         for (int d = 0; d < kQ; d++) {
-            if constexpr (PR::kShift && MODE == MODE_COLL) v[d] = mout[qaddr<L>(g, n, d)];
-            else v[d] = load_pop<PR>(src, qaddr<L>(g, n, d), d, rho0);
+            if constexpr (PR::kShift && MODE == MODE_COLL) v[d] = mout[qaddr<L>(g, qn, d)];
+            else v[d] = load_pop<PR, kStreamNT>(src, qaddr<L>(g, qn, d), d, rho0);
         }
     }
 
@@ -261,7 +295,7 @@ __device__ __forceinline__ void simple_cell(const typename PR::S* src,
         T m[kQ];
         es_moments<T, D>(v, m);
 #pragma unroll
-        for (int d = 0; d < kQ; d++) mout[qaddr<L>(g, n, d)] = m[d];
+        for (int d = 0; d < kQ; d++) mout[qaddr<L>(g, qn, d)] = m[d];
         return;
     }
 
@@ -271,7 +305,7 @@ __device__ __forceinline__ void simple_cell(const typename PR::S* src,
         fy = F[3 * (uint64_t)n + 1];
         fz = F[3 * (uint64_t)n + 2];
     }
-    T nuc = P.use_les ? Nu[n] : T(0);
+    T nuc = P.use_les ? ld<kStreamNT>(Nu + n) : T(0);
     T p[kQ];
     if (MODE == MODE_FUSED) {
         T m[kQ];
@@ -280,13 +314,13 @@ __device__ __forceinline__ void simple_cell(const typename PR::S* src,
     } else {
         es_collide<T, D>(v, fx, fy, fz, nuc, P, p);
     }
-    if (P.use_les) Nu[n] = nuc;
+    if (P.use_les) st<kStreamNT>(Nu + n, nuc);
 #pragma unroll
-    for (int d = 0; d < kQ; d++) store_pop<PR>(dst, qaddr<L>(g, n, d), d, p[d], rho0);
+    for (int d = 0; d < kQ; d++) store_pop<PR, kStreamNT>(dst, qaddr<L>(g, qn, d), d, p[d], rho0);
 }
 
 template <typename PR, int L, int MODE>
-__global__ __launch_bounds__(kBlock) void k_simple(const typename PR::S* src, typename PR::S* dst,
+__global__ __launch_bounds__(kRowBlock) void k_simple(const typename PR::S* src, typename PR::S* dst,
                                                    typename PR::T* mout, const typename PR::T* __restrict__ F,
                                                    typename PR::T* __restrict__ Nu,
                                                    const uint64_t* __restrict__ fmask, Grid g,
@@ -298,12 +332,6 @@ __global__ __launch_bounds__(kBlock) void k_simple(const typename PR::S* src, ty
 }
 
 // ------------------------------------------------------------------ data movement on every cell
-__device__ __forceinline__ void decode_cell(const Grid& g, uint64_t n, int& i, int& j, int& k) {
-    i = (int)(n / g.yz);
-    const uint32_t r = (uint32_t)(n - (uint64_t)i * g.yz);
-    j = (int)(r / (uint32_t)g.zg);
-    k = (int)(r - (uint32_t)j * (uint32_t)g.zg);
-}
 
 // push: dst[c][d] = src[c-e_d][d] if c-e_d is interior, else src[c][d] (all cells)
 template <typename W, int L>
@@ -312,11 +340,12 @@ __global__ void k_push_all(const W* __restrict__ src, W* __restrict__ dst, Grid 
          n += (uint64_t)gridDim.x * blockDim.x) {
         int i, j, k;
         decode_cell(g, n, i, j, k);
+        const uint32_t qc = qcell(g, i, j, k);
 #pragma unroll
         for (int d = 0; d < kQ; d++) {
             const int si = i - dir_x(d), sj = j - dir_y(d), sk = k - dir_z(d);
-            const uint32_t s = is_interior(g, si, sj, sk) ? cell_index(g, si, sj, sk) : (uint32_t)n;
-            dst[qaddr<L>(g, (uint32_t)n, d)] = src[qaddr<L>(g, s, d)];
+            const uint32_t s = is_interior(g, si, sj, sk) ? qcell(g, si, sj, sk) : qc;
+            dst[qaddr<L>(g, qc, d)] = src[qaddr<L>(g, s, d)];
         }
     }
 }
@@ -329,10 +358,11 @@ __global__ void k_pull_all(const W* __restrict__ src, W* __restrict__ dst, Grid 
         int i, j, k;
         decode_cell(g, n, i, j, k);
         const bool in = is_interior(g, i, j, k);
+        const uint32_t qc = qcell(g, i, j, k);
 #pragma unroll
         for (int d = 0; d < kQ; d++) {
-            const uint32_t s = in ? cell_index(g, i - dir_x(d), j - dir_y(d), k - dir_z(d)) : (uint32_t)n;
-            dst[qaddr<L>(g, (uint32_t)n, d)] = src[qaddr<L>(g, s, d)];
+            const uint32_t s = in ? qcell(g, i - dir_x(d), j - dir_y(d), k - dir_z(d)) : qc;
+            dst[qaddr<L>(g, qc, d)] = src[qaddr<L>(g, s, d)];
         }
     }
 }
@@ -356,12 +386,12 @@ __global__ void k_bb_face(W* __restrict__ Q, Grid g, int face) {
         if (axis == 0) { i = pos; j = a; k = b; }
         else if (axis == 1) { i = a; j = pos; k = b; }
         else { i = a; j = b; k = pos; }
-        const uint32_t c = cell_index(g, i, j, k);
+        const uint32_t c = qcell(g, i, j, k);
 #pragma unroll
         for (int d = 0; d < kQ; d++) {
             const int comp = axis == 0 ? dir_x(d) : axis == 1 ? dir_y(d) : dir_z(d);
             if (comp != inward) continue;
-            const uint32_t s = cell_index(g, i + dir_x(d), j + dir_y(d), k + dir_z(d));
+            const uint32_t s = qcell(g, i + dir_x(d), j + dir_y(d), k + dir_z(d));
             Q[qaddr<L>(g, c, d)] = Q[qaddr<L>(g, s, opposite(d))];
         }
     }
@@ -394,8 +424,8 @@ __global__ void k_bb_edges(W* __restrict__ Q, Grid g) {
         }
         const int lim = line < 4 ? g.zg1 : line < 8 ? g.yg1 : g.xg1;
         if (s > lim) continue;
-        const uint32_t c = cell_index(g, i, j, k);
-        const uint32_t src = cell_index(g, i + dir_x(d), j + dir_y(d), k + dir_z(d));
+        const uint32_t c = qcell(g, i, j, k);
+        const uint32_t src = qcell(g, i + dir_x(d), j + dir_y(d), k + dir_z(d));
         Q[qaddr<L>(g, c, d)] = Q[qaddr<L>(g, src, opposite(d))];
     }
 }
@@ -452,13 +482,13 @@ __global__ void k_ghost_fill(const typename PR::S* __restrict__ src, typename PR
         const int bj = (j == 0) ? 1 : (j == g.yg1 + 1) ? -1 : 0;
         const int bk = (k == 0) ? 1 : (k == g.zg1 + 1) ? -1 : 0;
         const int nb = (bi != 0) + (bj != 0) + (bk != 0);
-        const uint32_t c = cell_index(g, i, j, k);
+        const uint32_t c = qcell(g, i, j, k);
 #pragma unroll
         for (int d = 0; d < kQ; d++) {
             const int ex = dir_x(d), ey = dir_y(d), ez = dir_z(d);
             const int si = i - ex, sj = j - ey, sk = k - ez;
             if (is_interior(g, si, sj, sk)) {
-                dst[qaddr<L>(g, c, d)] = load_pop<PR>(src, qaddr<L>(g, cell_index(g, si, sj, sk), d), d, rho0);
+                dst[qaddr<L>(g, c, d)] = load_pop<PR>(src, qaddr<L>(g, qcell(g, si, sj, sk), d), d, rho0);
                 continue;
             }
             bool bb = false;
@@ -472,7 +502,7 @@ __global__ void k_ghost_fill(const typename PR::S* __restrict__ src, typename PR
             const int ti = i + ex, tj = j + ey, tk = k + ez;
             if (!is_interior(g, ti, tj, tk)) continue;
             // opp(d) has the rest weight of d
-            dst[qaddr<L>(g, c, d)] = load_pop<PR>(src, qaddr<L>(g, cell_index(g, ti, tj, tk), opposite(d)), d, rho0);
+            dst[qaddr<L>(g, c, d)] = load_pop<PR>(src, qaddr<L>(g, qcell(g, ti, tj, tk), opposite(d)), d, rho0);
         }
     }
 }
@@ -484,7 +514,7 @@ __global__ void k_ghost_copy(const W* __restrict__ src, W* __restrict__ dst, Gri
          t += (uint64_t)gridDim.x * blockDim.x) {
         int i, j, k;
         if (!ghost_cell(g, t, i, j, k)) continue;
-        const uint32_t c = cell_index(g, i, j, k);
+        const uint32_t c = qcell(g, i, j, k);
 #pragma unroll
         for (int d = 0; d < kQ; d++) dst[qaddr<L>(g, c, d)] = src[qaddr<L>(g, c, d)];
     }
@@ -584,10 +614,10 @@ __device__ __forceinline__ void cell_rho_j(const typename PR::S* __restrict__ Q,
                                            typename PR::T& rho, typename PR::T& jx, typename PR::T& jy,
                                            typename PR::T& jz, float rho0) {
     using T = typename PR::T;
-    const uint32_t n = cell_index(g, i, j, k);
+    const uint32_t n = qcell(g, i, j, k);
     if (STREAM == StreamEsotwist) {
-        const int32_t di = (i == g.xg1) ? -(int32_t)((uint32_t)(g.xg1 - 1) * g.yz) : (int32_t)g.yz;
-        const int32_t dj = (j == g.yg1) ? -(g.yg1 - 1) * g.zg : g.zg;
+        const int32_t di = (i == g.xg1) ? -(int32_t)((uint32_t)(g.xg1 - 1) * g.qyz) : (int32_t)g.qyz;
+        const int32_t dj = (j == g.yg1) ? -(int32_t)((uint32_t)(g.yg1 - 1) * g.qz) : (int32_t)g.qz;
         const int32_t dk = (k == g.zg1) ? -(g.zg1 - 1) : 1;
         if (MOMSRC == MOMSRC_STORED) {
             T m[4];
@@ -622,7 +652,7 @@ __device__ __forceinline__ void cell_rho_j(const typename PR::S* __restrict__ Q,
                 const int ex = dir_x(d), ey = dir_y(d), ez = dir_z(d);
                 const bool ok = (ex > 0 ? im : ex < 0 ? ip : true) && (ey > 0 ? jm : ey < 0 ? jp : true) &&
                                 (ez > 0 ? km : ez < 0 ? kp : true);
-                const int32_t off = -(ex * (int32_t)g.yz + ey * g.zg + ez);
+                const int32_t off = -(ex * (int32_t)g.qyz + ey * (int32_t)g.qz + ez);
                 f[d] = load_pop<PR>(Q, qaddr<L>(g, ok ? n + (uint32_t)off : n, d), d, rho0);
             }
             es_rho_j(f, rho, jx, jy, jz);
@@ -927,9 +957,12 @@ template <typename W, int L>
 __global__ void k_regrid_q(const W* __restrict__ old, W* __restrict__ nw, Grid og, Grid ng) {
     for (uint64_t n = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; n < ng.size;
          n += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t from = regrid_source(og, ng, n);
+        int i, j, k;
+        decode_cell(ng, n, i, j, k);
+        const uint32_t to = qcell(ng, i, j, k);
+        const uint32_t from = qcell(og, i / 2 + (i & 1), j / 2 + (j & 1), k / 2 + (k & 1));
 #pragma unroll
-        for (int l = 0; l < kQ; l++) nw[qaddr<L>(ng, (uint32_t)n, l)] = old[qaddr<L>(og, from, l)];
+        for (int l = 0; l < kQ; l++) nw[qaddr<L>(ng, to, l)] = old[qaddr<L>(og, from, l)];
     }
 }
 
@@ -960,7 +993,7 @@ __global__ void k_plane_xz(const T* __restrict__ Q, const T* __restrict__ F, con
         r.k = (uint16_t)(k0 + k - 1);
         r.pad = 0;
 #pragma unroll
-        for (int l = 0; l < 4; l++) r.q[l] = (float)Q[qaddr<L>(g, n, l)];
+        for (int l = 0; l < 4; l++) r.q[l] = (float)Q[qaddr<L>(g, qcell(g, i, cut, k), l)];
         q[t] = r;
         if (mask_test(fmask, g, i, cut, k)) {
             const T fx = F[3 * (uint64_t)n], fy = F[3 * (uint64_t)n + 1], fz = F[3 * (uint64_t)n + 2];
@@ -1087,9 +1120,12 @@ uint32_t grid_blocks(uint64_t n) {
     return (uint32_t)(b ? b : 1);
 }
 
-// rows of interior cells: blocks of kBlock threads along k
-inline void row_launch(const Grid& g, uint32_t& nkb, uint32_t& nblocks) {
-    nkb = (uint32_t)((g.zg1 + kBlock - 1) / kBlock);
+// rows of interior cells: nkb blocks of bk threads along k, bk a multiple of
+// the wavefront (64) covering the row in as few blocks of at most bmax lanes
+inline void row_launch(const Grid& g, uint32_t& nkb, uint32_t& nblocks, uint32_t& bk, int bmax) {
+    nkb = (uint32_t)((g.zg1 + bmax - 1) / bmax);
+    const uint32_t per = ((uint32_t)g.zg1 + nkb - 1) / nkb;
+    bk = (per + 63u) / 64u * 64u;
     nblocks = nkb * (uint32_t)g.yg1 * (uint32_t)g.xg1;
 }
 
@@ -1098,15 +1134,15 @@ hipError_t esotwist_t(int mode, int even, void* Q, void* mout, const void* F, vo
                       const Grid& g, const CollideArgs& c, float rho0, hipStream_t s) {
     using S = typename PR::S;
     using T = typename PR::T;
-    uint32_t nkb, nb;
-    row_launch(g, nkb, nb);
+    uint32_t nkb, nb, bk;
+    row_launch(g, nkb, nb, bk, row_block_max<S>());
     S* q = static_cast<S*>(Q);
     T* mo = static_cast<T*>(mout);
     const T* f = static_cast<const T*>(F);
     T* nu = static_cast<T*>(Nu);
     const auto P = to_params<T, typename PR::D>(c);
 #define TDLB_ESO(M, E) \
-    hipLaunchKernelGGL((k_esotwist<PR, L, M, E>), dim3(nb), dim3(kBlock), 0, s, q, mo, f, nu, fm, g, P, nkb, rho0)
+    hipLaunchKernelGGL((k_esotwist<PR, L, M, E>), dim3(nb), dim3(bk), 0, s, q, mo, f, nu, fm, g, P, nkb, rho0)
     if (mode == MODE_COLL) { if (even) TDLB_ESO(MODE_COLL, 1); else TDLB_ESO(MODE_COLL, 0); }
     else if (mode == MODE_MOM) { if (even) TDLB_ESO(MODE_MOM, 1); else TDLB_ESO(MODE_MOM, 0); }
     else if (mode == MODE_FUSED) { if (even) TDLB_ESO(MODE_FUSED, 1); else TDLB_ESO(MODE_FUSED, 0); }
@@ -1120,8 +1156,8 @@ hipError_t simple_t(int mode, const void* src, void* dst, void* mout, const void
                     const Grid& g, const CollideArgs& c, float rho0, hipStream_t s) {
     using S = typename PR::S;
     using T = typename PR::T;
-    uint32_t nkb, nb;
-    row_launch(g, nkb, nb);
+    uint32_t nkb, nb, bk;
+    row_launch(g, nkb, nb, bk, row_block_max<S>());
     const S* a = static_cast<const S*>(src);
     S* b = static_cast<S*>(dst);
     T* mo = static_cast<T*>(mout);
@@ -1129,7 +1165,7 @@ hipError_t simple_t(int mode, const void* src, void* dst, void* mout, const void
     T* nu = static_cast<T*>(Nu);
     const auto P = to_params<T, typename PR::D>(c);
 #define TDLB_SIM(M) \
-    hipLaunchKernelGGL((k_simple<PR, L, M>), dim3(nb), dim3(kBlock), 0, s, a, b, mo, f, nu, fm, g, P, nkb, rho0)
+    hipLaunchKernelGGL((k_simple<PR, L, M>), dim3(nb), dim3(bk), 0, s, a, b, mo, f, nu, fm, g, P, nkb, rho0)
     switch (mode) {
         case MODE_COLL: TDLB_SIM(MODE_COLL); break;
         case MODE_MOM: TDLB_SIM(MODE_MOM); break;
@@ -1177,7 +1213,7 @@ hipError_t force_level_t(int momsrc, int even, const PointsDev& p, int64_t begin
 
 // ====================================================================== launchers
 hipError_t launch_fill_q(int prec, int layout, void* Q, const Grid& g, double rho, hipStream_t s) {
-    const uint32_t nb = grid_blocks(g.size * kQ);
+    const uint32_t nb = grid_blocks((layout == LayoutIJKL ? g.size : g.qsize) * kQ);
     if (prec == 0) {
         if (layout == LayoutIJKL) hipLaunchKernelGGL((k_fill_q<float, LayoutIJKL>), dim3(nb), dim3(kBlock), 0, s, (float*)Q, g, (float)rho);
         else hipLaunchKernelGGL((k_fill_q<float, LayoutLIJK>), dim3(nb), dim3(kBlock), 0, s, (float*)Q, g, (float)rho);
@@ -1188,22 +1224,32 @@ hipError_t launch_fill_q(int prec, int layout, void* Q, const Grid& g, double rh
     return hipGetLastError();
 }
 
-hipError_t launch_relayout_q(int esize, int to, const void* src, void* dst, const Grid& g, hipStream_t s) {
+namespace {
+template <typename W>
+hipError_t q_boundary_t(int dlayout, int blayout, bool to_boundary, const void* src, void* dst, const Grid& g,
+                        hipStream_t s) {
     const uint32_t nb = grid_blocks(g.size * kQ);
-    if (esize == 4) {
-        if (to == LayoutLIJK)
-            hipLaunchKernelGGL((k_relayout_q<uint32_t, LayoutLIJK>), dim3(nb), dim3(kBlock), 0, s, (const uint32_t*)src, (uint32_t*)dst, g);
-        else
-            hipLaunchKernelGGL((k_relayout_q<uint32_t, LayoutIJKL>), dim3(nb), dim3(kBlock), 0, s, (const uint32_t*)src, (uint32_t*)dst, g);
-    } else if (esize == 8) {
-        if (to == LayoutLIJK)
-            hipLaunchKernelGGL((k_relayout_q<uint64_t, LayoutLIJK>), dim3(nb), dim3(kBlock), 0, s, (const uint64_t*)src, (uint64_t*)dst, g);
-        else
-            hipLaunchKernelGGL((k_relayout_q<uint64_t, LayoutIJKL>), dim3(nb), dim3(kBlock), 0, s, (const uint64_t*)src, (uint64_t*)dst, g);
-    } else {
-        return hipErrorInvalidValue;
-    }
+    const W* a = static_cast<const W*>(src);
+    W* b = static_cast<W*>(dst);
+#define TDLB_QB(DL, BL)                                                                                   \
+    do {                                                                                                  \
+        if (to_boundary) hipLaunchKernelGGL((k_q_boundary<W, DL, BL, true>), dim3(nb), dim3(kBlock), 0, s, a, b, g); \
+        else hipLaunchKernelGGL((k_q_boundary<W, DL, BL, false>), dim3(nb), dim3(kBlock), 0, s, a, b, g); \
+    } while (0)
+    if (dlayout == LayoutIJKL && blayout == LayoutIJKL) TDLB_QB(LayoutIJKL, LayoutIJKL);
+    else if (dlayout == LayoutIJKL) TDLB_QB(LayoutIJKL, LayoutLIJK);
+    else if (blayout == LayoutIJKL) TDLB_QB(LayoutLIJK, LayoutIJKL);
+    else TDLB_QB(LayoutLIJK, LayoutLIJK);
+#undef TDLB_QB
     return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_q_boundary(int esize, int dlayout, int blayout, int to_boundary, const void* src, void* dst,
+                             const Grid& g, hipStream_t s) {
+    if (esize == 4) return q_boundary_t<uint32_t>(dlayout, blayout, to_boundary != 0, src, dst, g, s);
+    if (esize == 8) return q_boundary_t<uint64_t>(dlayout, blayout, to_boundary != 0, src, dst, g, s);
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {

@@ -82,9 +82,17 @@ size_t footprint(const tdlb_server* s, int64_t edge) {
     const size_t pop = s->prec == TDLB_PREC_F64 ? 8 : s->prec == TDLB_PREC_F32 ? 4 : 2;
     const size_t t = s->prec == TDLB_PREC_F64 ? 8 : 4;
     const size_t nq = s->streaming == TDLB_STREAM_SIMPLE ? 2 : 1;
-    const size_t per_cell = nq * 18 * pop + 3 * t + t + 1;
-    const size_t masks = 5 * (size_t)(edge + 2) * (edge + 2) * ((edge + 2 + 63) / 64) * 8;
-    return (size_t)((double)(n * per_cell + masks) * 1.10);
+    // population cells as the handle lays them out
+    const int32_t zg = (int32_t)edge + 2;
+    Grid g{};
+    g.xg = g.yg = g.zg = zg;
+    g.xg1 = g.yg1 = g.zg1 = zg - 2;
+    g.yz = (uint32_t)(zg * zg);
+    g.size = n;
+    set_q_geometry(g, s->layout == TDLB_LAYOUT_IJKL ? LayoutIJKL : LayoutLIJK, pop);
+    const size_t qcells = (size_t)g.qsize;
+    const size_t masks = 5 * (size_t)zg * zg * ((zg + 63) / 64) * 8;
+    return (size_t)((double)(nq * 18 * pop * qcells + n * (3 * t + t + 1) + masks) * 1.10);
 }
 
 // The tank of a request: FlowParams as main.cpp leaves them (nu from
