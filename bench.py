@@ -299,6 +299,11 @@ def main():
                          "serve: a Poisson stream of tank requests; accuracy: fp32/fp16/bf16 against the fp64 "
                          "build (use --grid 96 --steps 5000)")
     ap.add_argument("--accuracy-re", type=float, nargs="+", default=[1.0e5, 7300.0])
+    ap.add_argument("--accuracy-streaming", nargs="+", default=["Esotwist"], choices=["Esotwist", "Simple"],
+                    help="accuracy: streamings (the Simple tank is unstable at these Reynolds numbers, as the "
+                         "reference's Simple wall treatment is)")
+    ap.add_argument("--accuracy-precisions", nargs="+", default=["fp32", "fp16", "bf16"],
+                    choices=["fp32", "fp16", "bf16"])
     ap.add_argument("--duration", type=float, default=600.0, help="serve: seconds of arrivals")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: arrivals per second")
     ap.add_argument("--tick-steps", type=int, default=50, help="serve: steps per scheduling tick")
@@ -336,14 +341,17 @@ def main():
     if args.workload == "accuracy":
         from tdlbcpp_amd import precision as P
         reports = {}
-        for name, st in (("Esotwist", td.STREAM_ESOTWIST), ("Simple", td.STREAM_SIMPLE)):
+        names = {"fp32": td.PREC_F32, "fp16": td.PREC_F16, "bf16": td.PREC_BF16}
+        precs = tuple(names[p] for p in args.accuracy_precisions)
+        for name in args.accuracy_streaming:
+            st = td.STREAM_ESOTWIST if name == "Esotwist" else td.STREAM_SIMPLE
             for re_ in args.accuracy_re:
-                r = P.precision_report(T=args.grid, steps=args.steps, precisions=(td.PREC_F32, td.PREC_F16,
-                                                                                 td.PREC_BF16),
-                                       streaming=st, re=re_, samples=20)
+                r = P.precision_report(T=args.grid, steps=args.steps, precisions=precs, streaming=st, re=re_,
+                                       samples=20)
                 reports[f"{name}/LIJK/Re{re_:g}"] = r
                 print(f"[bench] accuracy {name} Re={re_:g}: " + ", ".join(
-                    f"{k} L2 {v['velocity_l2']:.2e} Po err {v['po_rel_err']:.2e}" for k, v in r["variants"].items()),
+                    f"{k} L2 {v['velocity_l2']:.2e} (mean field {v['velocity_l2_mean']:.2e}) Po err "
+                    f"{v['po_rel_err']:.2e} mass drift {v['mass_drift']:.1e}" for k, v in r["variants"].items()),
                     file=sys.stderr, flush=True)
         print(json.dumps({"workload": f"accuracy_{args.grid}cubed_{args.steps}steps_vs_fp64", "reports": reports}),
               flush=True)

@@ -19,7 +19,7 @@
 // value (forcing's "F = 0 where O == 0", Forcing.hpp:141-151).
 //
 // Reduced-precision storage (TDLB_PREC_F16 / TDLB_PREC_BF16): the lazy phase
-// keeps the populations as 16-bit scaled deviations (f_d - w_d rho0) / w_d and
+// keeps the populations as 16-bit scaled deviations k_d (f_d - w_d rho0) and
 // computes in float; the explicit Q is a float array M (the FP32 reference
 // layout), written by materialise and read by the first collision of a run.
 // The per-call operations (collision, streaming, ...) need fp32 or fp64.

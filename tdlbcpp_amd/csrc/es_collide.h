@@ -58,9 +58,17 @@ template <>
 struct Div24<float, double> {
     TDLB_HD static float apply(float x) { return (float)((double)x * (1.0 / 24.0)); }
 };
+// Float-only arithmetic (the 16-bit storage formats): the product with
+// fl(1/24) is corrected by its exact residual x - 24 q, which gives the
+// correctly rounded quotient; the bare product is 3e-8 too large on average
+// and grew the mass of every cell by that much per step.
 template <>
 struct Div24<float, float> {
-    TDLB_HD static float apply(float x) { return x * (1.0f / 24.0f); }
+    TDLB_HD static float apply(float x) {
+        const float y = 1.0f / 24.0f;
+        const float q = x * y;
+        return fmaf(fmaf(-q, 24.0f, x), y, q);
+    }
 };
 
 // Eighteen moments of eighteen populations; same term order as the reference.
@@ -222,10 +230,18 @@ TDLB_HD inline T ib_weight(T xin) {
 }
 
 // Reduced-precision population storage: a population is kept as its deviation
-// from the rest population w_d*rho0, scaled by 1/w_d (w = 1/12 on the 6 faces,
-// 1/24 on the 12 edges, the rest populations of this lattice), so the stored
-// value is f/w_d - rho0: O(density and velocity fluctuations), well inside the
-// normal range of fp16 and bf16.  Arithmetic stays in FP32.
-TDLB_HD constexpr float rest_weight_inv(int d) { return d < 6 ? 12.0f : 24.0f; }
+// from the rest population w_d*rho0 (w = 1/12 on the 6 faces, 1/24 on the 12
+// edges, the rest populations of this lattice), scaled per direction by
+// k_d = 2 on the faces and 4 on the edges: s = k_d f - c0 with c0 = rho0/6
+// (k_d w_d = 1/6 in both), O(density and velocity fluctuations), inside the
+// normal range of fp16 and bf16.  k_d is a power of two and c0 sits on the
+// float grid of k_d f (both near 1/6), so decoding (s + c0) / k_d and encoding
+// k_d f - c0 are exact in float for every stored value near rest: the 16-bit
+// rounding is the only one.  (A 1/w_d scale multiplied every population by
+// 12 fl(1/12) = 1 + 3e-8 per step; scales 16/32 put fp16's finest steps below
+// the float grid of f, where the decode's ties round away from rest.)
+// Arithmetic stays in FP32.
+TDLB_HD constexpr float shift_scale(int d) { return d < 6 ? 2.0f : 4.0f; }
+TDLB_HD inline float shift_offset(float rho0) { return rho0 * (1.0f / 6.0f); }
 
 }  // namespace tdlb

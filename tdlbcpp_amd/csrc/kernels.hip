@@ -59,7 +59,7 @@ __device__ __forceinline__ uint32_t layout_block() {
 // ------------------------------------------------------------------ precisions
 // S: stored population type, T: arithmetic and F/Nu type, D: type the
 // reference's double literals promote to.  kShift: populations are stored as
-// scaled deviations from the rest populations (see rest_weight_inv).
+// scaled deviations from the rest populations (see shift_scale).
 struct PrecF32 {
     using S = float; using T = float; using D = double;
     static constexpr bool kShift = false;
@@ -68,13 +68,19 @@ struct PrecF64 {
     using S = double; using T = double; using D = double;
     static constexpr bool kShift = false;
 };
+// The 16-bit formats: enc rounds the scaled deviation to the stored type
+// (round to nearest even), dec widens it exactly.
 struct PrecF16 {
     using S = _Float16; using T = float; using D = float;
     static constexpr bool kShift = true;
+    __device__ static S enc(float s) { return (S)s; }
+    __device__ static float dec(S x) { return (float)x; }
 };
 struct PrecBF16 {
     using S = __bf16; using T = float; using D = float;
     static constexpr bool kShift = true;
+    __device__ static S enc(float s) { return (S)s; }
+    __device__ static float dec(S x) { return (float)x; }
 };
 // bit patterns moved unchanged
 template <typename W>
@@ -103,13 +109,13 @@ __device__ __forceinline__ void st(W* p, W v) {
 template <typename PR, bool NT = false>
 __device__ __forceinline__ typename PR::T load_pop(const typename PR::S* __restrict__ Q, uint64_t a, int d,
                                                    float rho0) {
-    if constexpr (PR::kShift) return ((float)ld<NT>(Q + a) + rho0) * (1.0f / rest_weight_inv(d));
+    if constexpr (PR::kShift) return (PR::dec(ld<NT>(Q + a)) + shift_offset(rho0)) * (1.0f / shift_scale(d));
     else return ld<NT>(Q + a);
 }
 template <typename PR, bool NT = false>
 __device__ __forceinline__ void store_pop(typename PR::S* __restrict__ Q, uint64_t a, int d, typename PR::T v,
                                           float rho0) {
-    if constexpr (PR::kShift) st<NT>(Q + a, (typename PR::S)(v * rest_weight_inv(d) - rho0));
+    if constexpr (PR::kShift) st<NT>(Q + a, PR::enc(v * shift_scale(d) - shift_offset(rho0)));
     else st<NT>(Q + a, v);
 }
 
@@ -1339,9 +1345,9 @@ hipError_t launch_ghost_fill(int esize, int layout, const void* src, void* dst, 
 hipError_t launch_ghost_fill_decode(int prec, int layout, const void* src, void* dst, const Grid& g, float rho0,
                                     hipStream_t s) {
     if (prec == 2)
-        TDLB_MOVE(k_ghost_fill, PrecF16, ghost_count(g), (const _Float16*)src, (float*)dst, g, rho0);
+        TDLB_MOVE(k_ghost_fill, PrecF16, ghost_count(g), (const PrecF16::S*)src, (float*)dst, g, rho0);
     else if (prec == 3)
-        TDLB_MOVE(k_ghost_fill, PrecBF16, ghost_count(g), (const __bf16*)src, (float*)dst, g, rho0);
+        TDLB_MOVE(k_ghost_fill, PrecBF16, ghost_count(g), (const PrecBF16::S*)src, (float*)dst, g, rho0);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
