@@ -94,7 +94,8 @@ def run_variant(td, hip, grid, streaming, layout, steps, warmup, precision=0, re
                collide_ms=coll_ms / max(ncoll, 1), forcing_ms=force_ms / max(nforce, 1),
                ib_points_static=n_static, ib_points_rotating=n_rot, forcing_levels=n_levels,
                setup_s=round(setup_s, 2))
-    res["bytes_per_cell"] = bytes_per_cell(POP_BYTES[precision], nu_bytes=8 if precision == 1 else 4)
+    res["bytes_per_cell"] = bytes_per_cell(POP_BYTES[precision], nu_bytes=POP_BYTES[precision] if precision >= 2
+                                           else 8 if precision == 1 else 4)
     res["collide_gbs"] = res["bytes_per_cell"] * cells / (res["collide_ms"] / 1e3) / 1e9
     cu.close()
     return res

@@ -120,7 +120,7 @@ MemberDev describe(const tdlb_batch* b, const Member& m) {
     md.Q[0] = cu->Q[0];
     md.Q[1] = cu->Q[1];
     md.F = static_cast<float*>(cu->F);
-    md.Nu = static_cast<float*>(cu->Nu);
+    md.Nu = cu->Nu;
     md.fmask[0] = cu->fmask[0];
     md.fmask[1] = cu->fmask[1];
     md.static_base = cu->static_base;
@@ -194,7 +194,7 @@ int tune_tile(tdlb_batch* b, const tdlb_cu* cu, int32_t* out) {
         md.Q[0] = q[0];
         md.Q[1] = q[nq - 1];
         md.F = static_cast<float*>(F);
-        md.Nu = static_cast<float*>(Nu);
+        md.Nu = Nu;
         md.fmask[0] = md.fmask[1] = static_cast<uint64_t*>(fm);
         md.active = 1;
         md.n_steps = 1;

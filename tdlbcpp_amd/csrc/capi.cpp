@@ -615,7 +615,7 @@ int tdlb_cu_field_bytes(tdlb_cu_t cu, int field, size_t* bytes) {
     switch (field) {
         case TDLB_FIELD_Q: *bytes = cu->qb_bytes(); break;
         case TDLB_FIELD_F: *bytes = cu->f_bytes(); break;
-        case TDLB_FIELD_NU: *bytes = cu->nu_bytes(); break;
+        case TDLB_FIELD_NU: *bytes = cu->nub_bytes(); break;
         case TDLB_FIELD_O: *bytes = cu->g.size; break;
         default: return fail(TDLB_ERR_INVALID, "field");
     }
@@ -648,6 +648,16 @@ int tdlb_cu_download(tdlb_cu_t cu, int field, void* host, size_t bytes) {
         if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("download Q: ") + hipGetErrorString(e));
         return TDLB_OK;
     }
+    if (field == TDLB_FIELD_NU && cu->shifted()) {
+        void* tmp = nullptr;
+        if (hipMalloc(&tmp, need) != hipSuccess) return fail(TDLB_ERR_NOMEM, "download Nu staging");
+        hipError_t e = launch_nu_convert(cu->prec(), 1, cu->Nu, tmp, cu->g.size, cu->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(host, tmp, need, hipMemcpyDeviceToHost, cu->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(cu->stream);
+        (void)hipFree(tmp);
+        if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("download Nu: ") + hipGetErrorString(e));
+        return TDLB_OK;
+    }
     const void* src = field == TDLB_FIELD_Q ? cu->eq() : field == TDLB_FIELD_NU ? cu->Nu : (const void*)cu->O;
     HIPCHK(hipMemcpyAsync(host, src, need, hipMemcpyDeviceToHost, cu->stream));
     HIPCHK(hipStreamSynchronize(cu->stream));
@@ -670,6 +680,16 @@ int tdlb_cu_upload(tdlb_cu_t cu, int field, const void* host, size_t bytes) {
         if (e == hipSuccess) e = hipStreamSynchronize(cu->stream);
         (void)hipFree(tmp);
         if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("upload Q: ") + hipGetErrorString(e));
+        return TDLB_OK;
+    }
+    if (field == TDLB_FIELD_NU && cu->shifted()) {
+        void* tmp = nullptr;
+        if (hipMalloc(&tmp, need) != hipSuccess) return fail(TDLB_ERR_NOMEM, "upload Nu staging");
+        hipError_t e = hipMemcpyAsync(tmp, host, need, hipMemcpyHostToDevice, cu->stream);
+        if (e == hipSuccess) e = launch_nu_convert(cu->prec(), 0, tmp, cu->Nu, cu->g.size, cu->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(cu->stream);
+        (void)hipFree(tmp);
+        if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("upload Nu: ") + hipGetErrorString(e));
         return TDLB_OK;
     }
     HIPCHK(hipMemcpyAsync(dst, host, need, hipMemcpyHostToDevice, cu->stream));
@@ -1131,7 +1151,8 @@ int tdlb_cu_double_resolution(tdlb_cu_t cu) {
     }
     hipError_t e = hipStreamSynchronize(cu->stream);
     if (e == hipSuccess)
-        e = launch_regrid((int)cu->tsize, cu->dlay(), cu->eq(), n->eq(), cu->Nu, n->Nu, cu->g, n->g, n->stream);
+        e = launch_regrid((int)cu->tsize, cu->dlay(), cu->eq(), n->eq(), (int)(cu->nu_bytes() / cu->g.size), cu->Nu,
+                          n->Nu, cu->g, n->g, n->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
     if (e != hipSuccess) return drop(fail(TDLB_ERR_HIP, std::string("regrid: ") + hipGetErrorString(e)));
     n->phase = PH_EXPLICIT;

@@ -217,6 +217,116 @@ TDLB_HD inline void es_collide(const T* m, T fx, T fy, T fz, T& nuc, const Colli
                  + o * (D)a[M10] - (D)a[M12] + (D)a[M14] - (D)a[M15] - (D)a[M16] + (D)a[M17] + (D)a[M18]);
 }
 
+// Moments and collision of one cell in float only, for the 16-bit storage
+// formats, which carry no bit-for-bit obligation: the same linear maps as
+// es_moments / es_collide (M17 and M18 are not formed: their relaxed values are
+// zero), factored through the sums and differences of opposite populations,
+// with multiply-adds fused.  Each relaxed moment is divided by 24 correctly
+// rounded (Div24<float, float>), so the density is not biased.  f in,
+// populations out (p may alias f); nuc as in es_collide.
+TDLB_HD inline void es_collide_fast(const float* f, float fx, float fy, float fz, float& nuc,
+                                    const CollideParams<float>& cp, float* p) {
+#pragma clang fp contract(fast)
+    // sums (s) and differences (a) of the nine opposite pairs
+    const float sx = f[Q01] + f[Q02], ax = f[Q01] - f[Q02];
+    const float sy = f[Q03] + f[Q04], ay = f[Q03] - f[Q04];
+    const float sz = f[Q05] + f[Q06], az = f[Q05] - f[Q06];
+    const float sxy = f[Q07] + f[Q08], axy = f[Q07] - f[Q08];    // +x+y / -x-y
+    const float sxz = f[Q09] + f[Q10], axz = f[Q09] - f[Q10];    // +x+z / -x-z
+    const float syz = f[Q11] + f[Q12], ayz = f[Q11] - f[Q12];    // +y+z / -y-z
+    const float sxmy = f[Q13] + f[Q14], axmy = f[Q13] - f[Q14];  // +x-y / -x+y
+    const float sxmz = f[Q15] + f[Q16], axmz = f[Q15] - f[Q16];  // +x-z / -x+z
+    const float symz = f[Q17] + f[Q18], aymz = f[Q17] - f[Q18];  // +y-z / -y+z
+
+    const float m01 = ((sx + sy) + (sz + sxy)) + ((sxz + syz) + (sxmy + sxmz)) + symz;
+    const float m02 = ax + (axy + axmy) + (axz + axmz);
+    const float m03 = ay + (axy - axmy) + (ayz + aymz);
+    const float m04 = az + (axz - axmz) + (ayz - aymz);
+    const float exy = sxy + sxmy, exz = sxz + sxmz, eyz = syz + symz;
+    const float m05 = (exy + exz) - (sy + sz);
+    const float m06 = sxy - sxmy;
+    const float m07 = (exy + eyz) - (sx + sz);
+    const float m08 = sxz - sxmz;
+    const float m09 = syz - symz;
+    const float m10 = (exz + eyz) - (sx + sy);
+    const float m11 = 2.0f * (axy + axmy) - (ax + axz + axmz);
+    const float m12 = 2.0f * (axy - axmy) - (ay + ayz + aymz);
+    const float m13 = 3.0f * ((axz + axmz) - ax);
+    const float m14 = 3.0f * ((ayz + aymz) - ay);
+    const float m15 = (axz - axmz) + (ayz - aymz) - 2.0f * az;
+    const float m16 = 3.0f * ((axmz - axz) + (ayz - aymz));
+
+    const float inv_rho = 1.0f / m01;
+    const float ux = inv_rho * (m02 + 0.5f * fx);
+    const float uy = inv_rho * (m03 + 0.5f * fy);
+    const float uz = inv_rho * (m04 + 0.5f * fz);
+    float b = cp.b0, c = cp.c0;
+    if (cp.use_les) {
+        const float fct = 3.0f / (m01 * (1.0f + 6.0f * (nuc + cp.nu)));
+        const float hfx = 0.5f * fx, hfy = 0.5f * fy, hfz = 0.5f * fz;
+        const float dudx = fct * ((m02 + hfx * ux) - m05);
+        const float dvdy = fct * ((m03 + hfy * uy) - m07);
+        const float dwdz = fct * ((m04 + hfz * uz) - m10);
+        const float divv = dudx + dvdy + dwdz;
+        const float fct2 = 2.0f * fct;
+        const float sh_xy = fct2 * ((m03 + hfy * ux) - m06);
+        const float sh_xz = fct2 * ((m04 + hfz * ux) - m08);
+        const float sh_yz = fct2 * ((m04 + hfz * uy) - m09);
+        const float sh = 2.0f * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + sh_xy * sh_xy + sh_xz * sh_xz +
+                         sh_yz * sh_yz - (2.0f / 3.0f) * divv * divv;
+        nuc = cp.cs0 * cp.cs0 * sqrt_abs(sh);
+        const float nut6 = 6.0f * (nuc + cp.nu);
+        b = 1.0f / (1.0f + nut6);
+        c = 1.0f - nut6;
+    }
+    const float jx2 = 2.0f * m02 + fx, jy2 = 2.0f * m03 + fy, jz2 = 2.0f * m04 + fz;
+    using Q = Div24<float, float>;
+    const float a01 = Q::apply(m01);
+    const float a02 = Q::apply(m02 + fx), a03 = Q::apply(m03 + fy), a04 = Q::apply(m04 + fz);
+    const float a05 = Q::apply((jx2 * ux - m05 * c) * b), a06 = Q::apply((jx2 * uy - m06 * c) * b);
+    const float a07 = Q::apply((jy2 * uy - m07 * c) * b), a08 = Q::apply((jx2 * uz - m08 * c) * b);
+    const float a09 = Q::apply((jy2 * uz - m09 * c) * b), a10 = Q::apply((jz2 * uz - m10 * c) * b);
+    const float ng3 = -cp.g3;
+    const float a11 = Q::apply(ng3 * m11), a12 = Q::apply(ng3 * m12), a13 = Q::apply(ng3 * m13);
+    const float a14 = Q::apply(ng3 * m14), a15 = Q::apply(ng3 * m15), a16 = Q::apply(ng3 * m16);
+
+    // faces: p = C +- D
+    const float two01 = 2.0f * a01;
+    const float cx = two01 + 3.0f * ((a05 - a07) - a10), dx = 4.0f * a02 - 2.0f * (a11 + a13);
+    const float cy = two01 + 3.0f * ((a07 - a05) - a10), dy = 4.0f * a03 - 2.0f * (a12 + a14);
+    const float cz = two01 + 3.0f * (a10 - (a05 + a07)), dz = 4.0f * (a04 - a15);
+    p[Q01] = cx + dx;
+    p[Q02] = cx - dx;
+    p[Q03] = cy + dy;
+    p[Q04] = cy - dy;
+    p[Q05] = cz + dz;
+    p[Q06] = cz - dz;
+    // edges: p = E +- 6 a_off +- G / H
+    const float exy2 = a01 + 1.5f * ((a05 + a07) - a10);
+    const float exz2 = a01 + 1.5f * ((a05 + a10) - a07);
+    const float eyz2 = a01 + 1.5f * ((a07 + a10) - a05);
+    const float gxy = 2.0f * ((a02 + a03) + (a11 + a12)), hxy = 2.0f * ((a02 - a03) + (a11 - a12));
+    const float gxz = 2.0f * (a02 + a04) + ((a13 - a11) + (a15 - a16));
+    const float hxz = 2.0f * (a02 - a04) + ((a13 - a11) + (a16 - a15));
+    const float gyz = 2.0f * (a03 + a04) + ((a14 - a12) + (a15 + a16));
+    const float hyz = 2.0f * (a03 - a04) + ((a14 - a12) - (a15 + a16));
+    const float pxy = exy2 + 6.0f * a06, mxy = exy2 - 6.0f * a06;
+    const float pxz = exz2 + 6.0f * a08, mxz = exz2 - 6.0f * a08;
+    const float pyz = eyz2 + 6.0f * a09, myz = eyz2 - 6.0f * a09;
+    p[Q07] = pxy + gxy;
+    p[Q08] = pxy - gxy;
+    p[Q13] = mxy + hxy;
+    p[Q14] = mxy - hxy;
+    p[Q09] = pxz + gxz;
+    p[Q10] = pxz - gxz;
+    p[Q15] = mxz + hxz;
+    p[Q16] = mxz - hxz;
+    p[Q11] = pyz + gyz;
+    p[Q12] = pyz - gyz;
+    p[Q17] = myz + hyz;
+    p[Q18] = myz - hyz;
+}
+
 // Smoothed-delta weight per axis (tdlbcpp/src/Forcing.hpp:18-35), evaluated in double.
 template <typename T>
 TDLB_HD inline T ib_weight(T xin) {

@@ -190,7 +190,8 @@ struct tdlb_cu {
                (size_t)g.qsize != (size_t)g.size;
     }
     size_t f_bytes() const { return (size_t)g.size * 3 * tsize; }
-    size_t nu_bytes() const { return (size_t)g.size * tsize; }
+    size_t nu_bytes() const { return (size_t)g.size * (shifted() ? ssize : tsize); }  // Nu on the device
+    size_t nub_bytes() const { return (size_t)g.size * tsize; }                       // Nu at the boundary
     size_t mask_bytes() const { return mask_words * sizeof(uint64_t); }
     bool simple() const { return prm.streaming == TDLB_STREAM_SIMPLE; }
     // layout of the lattice in HBM (what the kernels address) and of Q at the

@@ -1,7 +1,8 @@
 // Host-side launchers of the gfx950 kernels (implemented in kernels.hip).
 // Pointers are device pointers; every launcher enqueues on `s` and returns the
 // launch status.  prec: 0 float, 1 double, 2 fp16 storage, 3 bf16 storage (the
-// last two compute in float; F, Nu and point arrays are float).
+// last two compute in float and store Nu in their 16-bit format; F and point
+// arrays are float).
 // layout: LayoutIJKL/LayoutLIJK.
 #pragma once
 
@@ -63,9 +64,9 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // Populations in Q (stored type), moments in mout (arithmetic type, Q's shape
 // and addressing).  For prec 0/1 mout is Q itself.  For the reduced-precision
-// storage (prec 2 fp16, 3 bf16: populations kept as (f_d - w_d rho0) / w_d)
+// storage (prec 2 fp16, 3 bf16: populations kept as k_d (f_d - w_d rho0))
 // MODE_COLL reads the moments from mout and MODE_MOM / MODE_MOM_PULL write them
-// there; F and Nu are float.
+// there; F is float, Nu in the 16-bit format.
 hipError_t launch_esotwist(int prec, int layout, int mode, int even, void* Q, void* mout, const void* F, void* Nu,
                            const uint64_t* fmask, const Grid& g, const CollideArgs& c, float rho0, hipStream_t s);
 hipError_t launch_simple(int prec, int layout, int mode, const void* src, void* dst, void* mout, const void* F,
@@ -129,10 +130,13 @@ hipError_t launch_macroscopic(int prec, int layout, int streaming, int momsrc, i
                               const void* F, const uint64_t* fmask, const Grid& g, float rho0, void* out,
                               hipStream_t s);
 
-// doubleResolutionFullCU: Q (esize 4/8) and a per-cell array (esize 4/8) of the
-// old grid og copied into the doubled grid ng
-hipError_t launch_regrid(int esize, int layout, const void* oldQ, void* newQ, const void* oldNu, void* newNu,
-                         const Grid& og, const Grid& ng, hipStream_t s);
+// doubleResolutionFullCU: Q (esize 4/8) and a per-cell array (nu_esize 2/4/8)
+// of the old grid og copied into the doubled grid ng
+hipError_t launch_regrid(int esize, int layout, const void* oldQ, void* newQ, int nu_esize, const void* oldNu,
+                         void* newNu, const Grid& og, const Grid& ng, hipStream_t s);
+// n Nu values between the 16-bit storage of prec 2/3 and float (to_float != 0:
+// decode src into the float dst; else encode)
+hipError_t launch_nu_convert(int prec, int to_float, const void* src, void* dst, uint64_t n, hipStream_t s);
 
 // records of savePlaneXZ<float, 4> / <float, 3> (tDiskGrid, Output.hpp:22-25:
 // three uint16 then the floats at offset 8); the two alignment bytes are
@@ -178,7 +182,7 @@ struct MemberDev {
     int32_t tile;
     void* Q[2];  // Esotwist: Q[0]; Simple: the two lattices
     float* F;
-    float* Nu;
+    void* Nu;  // in the member's Nu storage type (float, or its 16-bit format)
     uint64_t* fmask[2];
     const uint64_t* static_base;
     PointsDev pts;  // static points [0, n_static), blades [n_static, n_static + n_rot)

@@ -57,35 +57,42 @@ __device__ __forceinline__ uint32_t layout_block() {
 }
 
 // ------------------------------------------------------------------ precisions
-// S: stored population type, T: arithmetic and F/Nu type, D: type the
-// reference's double literals promote to.  kShift: populations are stored as
-// scaled deviations from the rest populations (see shift_scale).
+// S: stored population type, T: arithmetic and F type, N: stored Nu type, D:
+// type the reference's double literals promote to.  kShift: populations are
+// stored as scaled deviations from the rest populations (see shift_scale) and
+// Nu in the same 16-bit format (enc_nu / dec_nu).
 struct PrecF32 {
-    using S = float; using T = float; using D = double;
+    using S = float; using T = float; using N = float; using D = double;
     static constexpr bool kShift = false;
 };
 struct PrecF64 {
-    using S = double; using T = double; using D = double;
+    using S = double; using T = double; using N = double; using D = double;
     static constexpr bool kShift = false;
 };
 // The 16-bit formats: enc rounds the scaled deviation to the stored type
-// (round to nearest even), dec widens it exactly.
+// (round to nearest even), dec widens it exactly.  Nu (>= 0, 1e-6 .. 1e-2 in a
+// stirred tank) is stored in the same type, scaled by 2^10 in fp16 so that it
+// stays above fp16's smallest normal (6.1e-5) down to 6e-8.
 struct PrecF16 {
-    using S = _Float16; using T = float; using D = float;
+    using S = _Float16; using T = float; using N = _Float16; using D = float;
     static constexpr bool kShift = true;
     __device__ static S enc(float s) { return (S)s; }
     __device__ static float dec(S x) { return (float)x; }
+    __host__ __device__ static N enc_nu(float v) { return (N)(v * 1024.0f); }
+    __host__ __device__ static float dec_nu(N x) { return (float)x * (1.0f / 1024.0f); }
 };
 struct PrecBF16 {
-    using S = __bf16; using T = float; using D = float;
+    using S = __bf16; using T = float; using N = __bf16; using D = float;
     static constexpr bool kShift = true;
     __device__ static S enc(float s) { return (S)s; }
     __device__ static float dec(S x) { return (float)x; }
+    __host__ __device__ static N enc_nu(float v) { return (N)v; }
+    __host__ __device__ static float dec_nu(N x) { return (float)x; }
 };
 // bit patterns moved unchanged
 template <typename W>
 struct RawBits {
-    using S = W; using T = W; using D = W;
+    using S = W; using T = W; using N = W; using D = W;
     static constexpr bool kShift = false;
 };
 
@@ -109,13 +116,26 @@ __device__ __forceinline__ void st(W* p, W v) {
 template <typename PR, bool NT = false>
 __device__ __forceinline__ typename PR::T load_pop(const typename PR::S* __restrict__ Q, uint64_t a, int d,
                                                    float rho0) {
-    if constexpr (PR::kShift) return (PR::dec(ld<NT>(Q + a)) + shift_offset(rho0)) * (1.0f / shift_scale(d));
+    // (dec + c0) / k as one multiply-add: x / k and c0 / k are exact (k a power of two)
+    if constexpr (PR::kShift)
+        return fmaf(PR::dec(ld<NT>(Q + a)), 1.0f / shift_scale(d), shift_offset(rho0) * (1.0f / shift_scale(d)));
     else return ld<NT>(Q + a);
+}
+template <typename PR, bool NT = false>
+__device__ __forceinline__ typename PR::T load_nu(const typename PR::N* __restrict__ Nu, uint64_t n) {
+    if constexpr (PR::kShift) return PR::dec_nu(ld<NT>(Nu + n));
+    else return ld<NT>(Nu + n);
+}
+template <typename PR, bool NT = false>
+__device__ __forceinline__ void store_nu(typename PR::N* __restrict__ Nu, uint64_t n, typename PR::T v) {
+    if constexpr (PR::kShift) st<NT>(Nu + n, PR::enc_nu(v));
+    else st<NT>(Nu + n, v);
 }
 template <typename PR, bool NT = false>
 __device__ __forceinline__ void store_pop(typename PR::S* __restrict__ Q, uint64_t a, int d, typename PR::T v,
                                           float rho0) {
-    if constexpr (PR::kShift) st<NT>(Q + a, PR::enc(v * shift_scale(d) - shift_offset(rho0)));
+    // k v - c0 as one multiply-add (k v is exact)
+    if constexpr (PR::kShift) st<NT>(Q + a, PR::enc(fmaf(v, shift_scale(d), -shift_offset(rho0))));
     else st<NT>(Q + a, v);
 }
 
@@ -183,7 +203,7 @@ __global__ void k_copy_bytes(uint8_t* __restrict__ d, const uint8_t* __restrict_
 // are stored as T, a separate T array in the reference layout otherwise.
 template <typename PR, int L, int MODE, int EVEN>
 __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, typename PR::T* mout,
-                                              const typename PR::T* __restrict__ F, typename PR::T* __restrict__ Nu,
+                                              const typename PR::T* __restrict__ F, typename PR::N* __restrict__ Nu,
                                               const uint64_t* __restrict__ fmask, const Grid& g,
                                               const CollideParams<typename PR::T>& P, float rho0, int i, int j,
                                               int k) {
@@ -223,16 +243,18 @@ __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, ty
         fy = F[3 * (uint64_t)n + 1];
         fz = F[3 * (uint64_t)n + 2];
     }
-    T nuc = P.use_les ? ld<kStreamNT>(Nu + n) : T(0);
+    T nuc = P.use_les ? load_nu<PR, kStreamNT>(Nu, n) : T(0);
     T p[kQ];
-    if (MODE == MODE_FUSED) {
+    if constexpr (MODE == MODE_FUSED && PR::kShift) {
+        es_collide_fast(v, fx, fy, fz, nuc, P, p);
+    } else if (MODE == MODE_FUSED) {
         T m[kQ];
         es_moments<T, D>(v, m);
         es_collide<T, D>(m, fx, fy, fz, nuc, P, p);
     } else {
         es_collide<T, D>(v, fx, fy, fz, nuc, P, p);
     }
-    if (P.use_les) st<kStreamNT>(Nu + n, nuc);
+    if (P.use_les) store_nu<PR, kStreamNT>(Nu, n, nuc);
     // Recompute the 18 home cells instead of keeping them live through the
     // collision: the opaque copy stops the compiler from reusing the load
     // addresses, which frees ~36 VGPRs and raises occupancy.
@@ -250,7 +272,7 @@ __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, ty
 template <typename PR, int L, int MODE, int EVEN>
 __global__ __launch_bounds__(kRowBlock) void k_esotwist(typename PR::S* __restrict__ Q, typename PR::T* mout,
                                                      const typename PR::T* __restrict__ F,
-                                                     typename PR::T* __restrict__ Nu,
+                                                     typename PR::N* __restrict__ Nu,
                                                      const uint64_t* __restrict__ fmask, Grid g,
                                                      CollideParams<typename PR::T> P, uint32_t nkb, float rho0) {
     const RowBlock rb = row_block(g, layout_block<L>(), nkb, blockDim.x);
@@ -267,7 +289,7 @@ template <typename PR, int L, int MODE>
 __device__ __forceinline__ void simple_cell(const typename PR::S* src,
                                             typename PR::S* dst,   // equal to src for in-place modes
                                             typename PR::T* mout,  // moments output of the MOM modes
-                                            const typename PR::T* __restrict__ F, typename PR::T* __restrict__ Nu,
+                                            const typename PR::T* __restrict__ F, typename PR::N* __restrict__ Nu,
                                             const uint64_t* __restrict__ fmask, const Grid& g,
                                             const CollideParams<typename PR::T>& P, float rho0, int i, int j,
                                             int k) {
@@ -311,16 +333,18 @@ __device__ __forceinline__ void simple_cell(const typename PR::S* src,
         fy = F[3 * (uint64_t)n + 1];
         fz = F[3 * (uint64_t)n + 2];
     }
-    T nuc = P.use_les ? ld<kStreamNT>(Nu + n) : T(0);
+    T nuc = P.use_les ? load_nu<PR, kStreamNT>(Nu, n) : T(0);
     T p[kQ];
-    if (MODE == MODE_FUSED) {
+    if constexpr (MODE == MODE_FUSED && PR::kShift) {
+        es_collide_fast(v, fx, fy, fz, nuc, P, p);
+    } else if (MODE == MODE_FUSED) {
         T m[kQ];
         es_moments<T, D>(v, m);
         es_collide<T, D>(m, fx, fy, fz, nuc, P, p);
     } else {
         es_collide<T, D>(v, fx, fy, fz, nuc, P, p);
     }
-    if (P.use_les) st<kStreamNT>(Nu + n, nuc);
+    if (P.use_les) store_nu<PR, kStreamNT>(Nu, n, nuc);
 #pragma unroll
     for (int d = 0; d < kQ; d++) store_pop<PR, kStreamNT>(dst, qaddr<L>(g, qn, d), d, p[d], rho0);
 }
@@ -328,7 +352,7 @@ __device__ __forceinline__ void simple_cell(const typename PR::S* src,
 template <typename PR, int L, int MODE>
 __global__ __launch_bounds__(kRowBlock) void k_simple(const typename PR::S* src, typename PR::S* dst,
                                                    typename PR::T* mout, const typename PR::T* __restrict__ F,
-                                                   typename PR::T* __restrict__ Nu,
+                                                   typename PR::N* __restrict__ Nu,
                                                    const uint64_t* __restrict__ fmask, Grid g,
                                                    CollideParams<typename PR::T> P, uint32_t nkb, float rho0) {
     const RowBlock rb = row_block(g, layout_block<L>(), nkb, blockDim.x);
@@ -1145,7 +1169,7 @@ hipError_t esotwist_t(int mode, int even, void* Q, void* mout, const void* F, vo
     S* q = static_cast<S*>(Q);
     T* mo = static_cast<T*>(mout);
     const T* f = static_cast<const T*>(F);
-    T* nu = static_cast<T*>(Nu);
+    auto* nu = static_cast<typename PR::N*>(Nu);
     const auto P = to_params<T, typename PR::D>(c);
 #define TDLB_ESO(M, E) \
     hipLaunchKernelGGL((k_esotwist<PR, L, M, E>), dim3(nb), dim3(bk), 0, s, q, mo, f, nu, fm, g, P, nkb, rho0)
@@ -1168,7 +1192,7 @@ hipError_t simple_t(int mode, const void* src, void* dst, void* mout, const void
     S* b = static_cast<S*>(dst);
     T* mo = static_cast<T*>(mout);
     const T* f = static_cast<const T*>(F);
-    T* nu = static_cast<T*>(Nu);
+    auto* nu = static_cast<typename PR::N*>(Nu);
     const auto P = to_params<T, typename PR::D>(c);
 #define TDLB_SIM(M) \
     hipLaunchKernelGGL((k_simple<PR, L, M>), dim3(nb), dim3(bk), 0, s, a, b, mo, f, nu, fm, g, P, nkb, rho0)
@@ -1553,21 +1577,52 @@ hipError_t launch_macroscopic(int prec, int layout, int streaming, int momsrc, i
     return e;
 }
 
-hipError_t launch_regrid(int esize, int layout, const void* oldQ, void* newQ, const void* oldNu, void* newNu,
-                         const Grid& og, const Grid& ng, hipStream_t s) {
+hipError_t launch_regrid(int esize, int layout, const void* oldQ, void* newQ, int nu_esize, const void* oldNu,
+                         void* newNu, const Grid& og, const Grid& ng, hipStream_t s) {
     const uint32_t nb = grid_blocks(ng.size);
     if (esize == 4) {
         if (layout == LayoutIJKL)
             hipLaunchKernelGGL((k_regrid_q<uint32_t, LayoutIJKL>), dim3(nb), dim3(kBlock), 0, s, (const uint32_t*)oldQ, (uint32_t*)newQ, og, ng);
         else
             hipLaunchKernelGGL((k_regrid_q<uint32_t, LayoutLIJK>), dim3(nb), dim3(kBlock), 0, s, (const uint32_t*)oldQ, (uint32_t*)newQ, og, ng);
-        hipLaunchKernelGGL(k_regrid_cells<uint32_t>, dim3(nb), dim3(kBlock), 0, s, (const uint32_t*)oldNu, (uint32_t*)newNu, og, ng);
     } else if (esize == 8) {
         if (layout == LayoutIJKL)
             hipLaunchKernelGGL((k_regrid_q<uint64_t, LayoutIJKL>), dim3(nb), dim3(kBlock), 0, s, (const uint64_t*)oldQ, (uint64_t*)newQ, og, ng);
         else
             hipLaunchKernelGGL((k_regrid_q<uint64_t, LayoutLIJK>), dim3(nb), dim3(kBlock), 0, s, (const uint64_t*)oldQ, (uint64_t*)newQ, og, ng);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    if (nu_esize == 2)
+        hipLaunchKernelGGL(k_regrid_cells<uint16_t>, dim3(nb), dim3(kBlock), 0, s, (const uint16_t*)oldNu, (uint16_t*)newNu, og, ng);
+    else if (nu_esize == 4)
+        hipLaunchKernelGGL(k_regrid_cells<uint32_t>, dim3(nb), dim3(kBlock), 0, s, (const uint32_t*)oldNu, (uint32_t*)newNu, og, ng);
+    else if (nu_esize == 8)
         hipLaunchKernelGGL(k_regrid_cells<uint64_t>, dim3(nb), dim3(kBlock), 0, s, (const uint64_t*)oldNu, (uint64_t*)newNu, og, ng);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+namespace {
+// Nu between the 16-bit storage of prec 2/3 and the float array of the boundary
+template <typename PR, bool TO_FLOAT>
+__global__ void k_nu_convert(const void* __restrict__ src, void* __restrict__ dst, uint64_t n) {
+    for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < n; a += (uint64_t)gridDim.x * blockDim.x) {
+        if (TO_FLOAT) static_cast<float*>(dst)[a] = PR::dec_nu(static_cast<const typename PR::N*>(src)[a]);
+        else static_cast<typename PR::N*>(dst)[a] = PR::enc_nu(static_cast<const float*>(src)[a]);
+    }
+}
+}  // namespace
+
+hipError_t launch_nu_convert(int prec, int to_float, const void* src, void* dst, uint64_t n, hipStream_t s) {
+    const uint32_t nb = grid_blocks(n);
+    if (prec == 2) {
+        if (to_float) hipLaunchKernelGGL((k_nu_convert<PrecF16, true>), dim3(nb), dim3(kBlock), 0, s, src, dst, n);
+        else hipLaunchKernelGGL((k_nu_convert<PrecF16, false>), dim3(nb), dim3(kBlock), 0, s, src, dst, n);
+    } else if (prec == 3) {
+        if (to_float) hipLaunchKernelGGL((k_nu_convert<PrecBF16, true>), dim3(nb), dim3(kBlock), 0, s, src, dst, n);
+        else hipLaunchKernelGGL((k_nu_convert<PrecBF16, false>), dim3(nb), dim3(kBlock), 0, s, src, dst, n);
     } else {
         return hipErrorInvalidValue;
     }
@@ -1682,11 +1737,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_fused(const MemberDev* __restr
     const uint64_t* fm = md.fmask[md.fm];
     if constexpr (STREAM == StreamEsotwist) {
         S* Q = static_cast<S*>(md.Q[0]);
-        if (md.even) esotwist_cell<PR, L, MODE_FUSED, 1>(Q, nullptr, md.F, md.Nu, fm, md.g, md.P, md.rho0, i, j, k);
-        else esotwist_cell<PR, L, MODE_FUSED, 0>(Q, nullptr, md.F, md.Nu, fm, md.g, md.P, md.rho0, i, j, k);
+        auto* nu = static_cast<typename PR::N*>(md.Nu);
+        if (md.even) esotwist_cell<PR, L, MODE_FUSED, 1>(Q, nullptr, md.F, nu, fm, md.g, md.P, md.rho0, i, j, k);
+        else esotwist_cell<PR, L, MODE_FUSED, 0>(Q, nullptr, md.F, nu, fm, md.g, md.P, md.rho0, i, j, k);
     } else {
         simple_cell<PR, L, MODE_FUSED>(static_cast<const S*>(md.Q[md.cur]), static_cast<S*>(md.Q[1 - md.cur]),
-                                       nullptr, md.F, md.Nu, fm, md.g, md.P, md.rho0, i, j, k);
+                                       nullptr, md.F, static_cast<typename PR::N*>(md.Nu), fm, md.g, md.P,
+                                       md.rho0, i, j, k);
     }
 }
 

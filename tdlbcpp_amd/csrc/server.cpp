@@ -92,7 +92,7 @@ size_t footprint(const tdlb_server* s, int64_t edge) {
     set_q_geometry(g, s->layout == TDLB_LAYOUT_IJKL ? LayoutIJKL : LayoutLIJK, pop);
     const size_t qcells = (size_t)g.qsize;
     const size_t masks = 5 * (size_t)zg * zg * ((zg + 63) / 64) * 8;
-    return (size_t)((double)(nq * 18 * pop * qcells + n * (3 * t + t + 1) + masks) * 1.10);
+    return (size_t)((double)(nq * 18 * pop * qcells + n * (3 * t + (pop == 2 ? 2 : t) + 1) + masks) * 1.10);
 }
 
 // The tank of a request: FlowParams as main.cpp leaves them (nu from
