@@ -120,7 +120,7 @@ def cpu_baseline(sample_grid=96, max_seconds=20.0):
         loop.run(1)
         steps += 1
         el = time.time() - t0
-        if el > max_seconds or steps >= 50:
+        if el > max_seconds or steps >= 500:
             break
     del np
     return dict(value=round(sample_grid ** 3 * steps / el / 1e6, 3), unit="MLUPS", cores=1, kind="port",
@@ -294,7 +294,8 @@ def main():
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--variants", default="all", help="all, precision (fp32 + fp16/bf16 Esotwist/LIJK) or headline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="seconds of oracle main-loop steps timed for cpu_baseline")
     ap.add_argument("--workload", default="tank", choices=["tank", "batch", "serve", "accuracy"],
                     help="tank: one 512^3 tank (headline); batch: independent tanks batched vs one at a time; "
                          "serve: a Poisson stream of tank requests; accuracy: fp32/fp16/bf16 against the fp64 "
