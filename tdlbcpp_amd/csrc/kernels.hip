@@ -25,14 +25,14 @@ namespace tdlb {
 namespace {
 
 constexpr int kBlock = 256;
-// Stream-collide kernels with 16-bit storage take one block per interior k-row
-// when the row fits (up to kRowBlock lanes), so a 128 B line of a slot plane is
-// read by one block only, except the line two neighbouring rows share (measured
-// 3-4 % faster at 512^3 than 256-lane blocks).  32- and 64-bit storage keep
-// 256-lane blocks, which measured 2-19 % faster than 512 there.
+// Stream-collide kernels take one block per interior k-row when the row fits
+// (up to kRowBlock lanes).  The population arrays keep interior cells only in
+// their rows (see device.h), so a block's run of each slot plane is whole
+// lines that no other block touches; at 512^3 one 512-lane block per row
+// measured 4 % faster than two 256-lane blocks in FP32 and 3-4 % in 16 bits.
 constexpr int kRowBlock = 512;
 template <typename S>
-constexpr int row_block_max() { return sizeof(S) == 2 ? kRowBlock : kBlock; }
+constexpr int row_block_max() { return kRowBlock; }
 
 inline uint32_t blocks_for(uint64_t n, int bs = kBlock) {
     uint64_t b = (n + bs - 1) / bs;
