@@ -9,8 +9,9 @@
 // are evaluated in double and rounded back to T; every expression below spells
 // those promotions out as (D) casts and keeps the reference's left-to-right order
 // of terms, so that results are identical to the reference.  With P = float the
-// same expressions run entirely in FP32 (the reduced-precision storage modes).
-// All code that includes this header is compiled with -ffp-contract=off.
+// same expressions run entirely in FP32.  All code that includes this header
+// is compiled with -ffp-contract=off; es_collide_fast, the float-only collision
+// of the 16-bit storage formats, alone turns contraction on for its body.
 //
 // One deliberate rewrite: the reference divides each of the 18 relaxed moments
 // by 24.0 in double and rounds to float.  For T = float we multiply by the
