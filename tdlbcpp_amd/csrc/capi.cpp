@@ -401,7 +401,10 @@ int tdlb_cu_create(const tdlb_cu_params* p, const tdlb_flow* f, tdlb_cu_t* out) 
     if (f->collision != 0) return fail(TDLB_ERR_UNSUPPORTED, "only the EgglesSomers collision is implemented");
     const int64_t xg = p->x + 2, yg = p->y + 2, zg = p->z + 2;
     const uint64_t size = (uint64_t)xg * yg * zg;
-    if (size >= (1ull << 32) / kQ) return fail(TDLB_ERR_UNSUPPORTED, "grid too large for 32-bit cell indexing");
+    // cell indices and the Esotwist wrap offsets are 32-bit (slot addresses are
+    // 64-bit): up to 2^31 cells incl. ghosts, a 1288^3 grid
+    if (size >= (1ull << 31))
+        return fail(TDLB_ERR_UNSUPPORTED, "grid too large: at most 2^31 cells including ghosts");
 
     tdlb_cu* cu = new tdlb_cu();
     cu->prm = *p;

@@ -1,0 +1,42 @@
+"""A tank past 2^32 / 18 cells, the size where an IJKL slot address (cell * 18 +
+slot) no longer fits 32 bits and the LIJK slot planes lie more than 4 GiB apart
+in total.  The IJKL and LIJK kernels address the same state differently; they
+agree bit for bit at small sizes (test_gpu_parity.py), so they must agree here
+too, through the deterministic FP64 sums of the whole state, and Esotwist must
+hold the mass.
+"""
+import sys
+
+import pytest
+
+import tdlbcpp_amd as td
+
+
+def case(msg):
+    print(f"[gpu case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.gpu
+def test_large_tank_ijkl_and_lijk_agree():
+    T, steps = 650, 3  # 652^3 = 277 M cells incl. ghosts > 2^32 / 18 = 238.6 M
+    assert (T + 2) ** 3 * 18 > 2 ** 32
+    flow = td.flow_for_tank(T, re=1.0e5, uav=0.1, cs0=0.1, use_les=1)
+    sums = []
+    for layout, name in ((td.LAYOUT_LIJK, "LIJK"), (td.LAYOUT_IJKL, "IJKL")):
+        case(f"large tank {T}^3 Esotwist {name} fp32, {steps} steps")
+        cu = td.ComputeUnit(T, layout=layout, streaming=td.STREAM_ESOTWIST, precision=td.PREC_F32, **flow)
+        try:
+            cu.initialise(1.0)
+            cu.bind_rushton(td.rushton_defaults(T, tip_speed=0.1, startup_steps=10))
+            m0 = cu.reductions()["mass"]
+            cu.run(steps)
+            r = cu.reductions()
+        finally:
+            cu.close()
+        print(f"[gpu case] {name}: mass {r['mass']:.6f} (start {m0:.6f}) momentum {r['momentum']} "
+              f"torque {r['torque']:.6e}", flush=True)
+        assert abs(r["mass"] / m0 - 1.0) < 1e-6
+        assert r["torque"] != 0.0
+        sums.append(r)
+    assert sums[0] == sums[1]
