@@ -65,7 +65,7 @@ def bytes_per_cell(pop_bytes=4, les=True, nu_bytes=4):
     return 2 * 18 * pop_bytes + (2 * nu_bytes if les else 0)
 
 
-POP_BYTES = {0: 4, 1: 8, 2: 2, 3: 2}
+POP_BYTES = {0: 4, 1: 8, 2: 2, 3: 2, 4: 2}
 
 
 def run_variant(td, hip, grid, streaming, layout, steps, warmup, precision=0, re=1.0e5, uav=0.1):
@@ -292,20 +292,20 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--grid", type=int, default=512)
-    ap.add_argument("--variants", default="all", help="all, precision (fp32 + fp16/bf16 Esotwist/LIJK) or headline")
+    ap.add_argument("--variants", default="all", help="all, precision (fp32 + fp16/bf16/e3m12 Esotwist/LIJK) or headline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="seconds of oracle main-loop steps timed for cpu_baseline")
     ap.add_argument("--workload", default="tank", choices=["tank", "batch", "serve", "accuracy"],
                     help="tank: one 512^3 tank (headline); batch: independent tanks batched vs one at a time; "
-                         "serve: a Poisson stream of tank requests; accuracy: fp32/fp16/bf16 against the fp64 "
+                         "serve: a Poisson stream of tank requests; accuracy: fp32/fp16/bf16/e3m12 against the fp64 "
                          "build (use --grid 96 --steps 5000)")
     ap.add_argument("--accuracy-re", type=float, nargs="+", default=[1.0e5, 7300.0])
     ap.add_argument("--accuracy-streaming", nargs="+", default=["Esotwist"], choices=["Esotwist", "Simple"],
                     help="accuracy: streamings (the Simple tank is unstable at these Reynolds numbers, as the "
                          "reference's Simple wall treatment is)")
-    ap.add_argument("--accuracy-precisions", nargs="+", default=["fp32", "fp16", "bf16"],
-                    choices=["fp32", "fp16", "bf16"])
+    ap.add_argument("--accuracy-precisions", nargs="+", default=["fp32", "fp16", "bf16", "e3m12"],
+                    choices=["fp32", "fp16", "bf16", "e3m12"])
     ap.add_argument("--duration", type=float, default=600.0, help="serve: seconds of arrivals")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: arrivals per second")
     ap.add_argument("--tick-steps", type=int, default=50, help="serve: steps per scheduling tick")
@@ -343,7 +343,7 @@ def main():
     if args.workload == "accuracy":
         from tdlbcpp_amd import precision as P
         reports = {}
-        names = {"fp32": td.PREC_F32, "fp16": td.PREC_F16, "bf16": td.PREC_BF16}
+        names = {"fp32": td.PREC_F32, "fp16": td.PREC_F16, "bf16": td.PREC_BF16, "e3m12": td.PREC_E3M12}
         precs = tuple(names[p] for p in args.accuracy_precisions)
         for name in args.accuracy_streaming:
             st = td.STREAM_ESOTWIST if name == "Esotwist" else td.STREAM_SIMPLE
@@ -370,11 +370,12 @@ def main():
                "config": {"workload": "served_rushton_tanks_fp16_esotwist_lijk", **r}}
         print(json.dumps(out), flush=True)
         return 0
-    F32, F16, BF16 = td.PREC_F32, td.PREC_F16, td.PREC_BF16
+    F32, F16, BF16, E3M12 = td.PREC_F32, td.PREC_F16, td.PREC_BF16, td.PREC_E3M12
     variants = [("Esotwist/LIJK", td.STREAM_ESOTWIST, td.LAYOUT_LIJK, F32)]
     if args.variants in ("all", "precision"):
         variants += [("Esotwist/LIJK/fp16", td.STREAM_ESOTWIST, td.LAYOUT_LIJK, F16),
-                     ("Esotwist/LIJK/bf16", td.STREAM_ESOTWIST, td.LAYOUT_LIJK, BF16)]
+                     ("Esotwist/LIJK/bf16", td.STREAM_ESOTWIST, td.LAYOUT_LIJK, BF16),
+                     ("Esotwist/LIJK/e3m12", td.STREAM_ESOTWIST, td.LAYOUT_LIJK, E3M12)]
     if args.variants == "all":
         variants += [("Esotwist/IJKL", td.STREAM_ESOTWIST, td.LAYOUT_IJKL, F32),
                      ("Simple/LIJK", td.STREAM_SIMPLE, td.LAYOUT_LIJK, F32),
@@ -406,6 +407,8 @@ def main():
                    "reynolds": 1.0e5, "tip_speed": 0.1, "variants": results},
         "fp16_store_speedup": (round(results["Esotwist/LIJK/fp16"]["mlups"] / head["mlups"], 3)
                                if "Esotwist/LIJK/fp16" in results else None),
+        "e3m12_store_speedup": (round(results["Esotwist/LIJK/e3m12"]["mlups"] / head["mlups"], 3)
+                                if "Esotwist/LIJK/e3m12" in results else None),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_esotwist<float,LIJK,FUSED>",

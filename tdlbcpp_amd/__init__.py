@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libtdlb_hip.so")
 
 LAYOUT_IJKL, LAYOUT_LIJK, LAYOUT_IJKL_SOA = 0, 1, 2  # IJKL_SOA: IJKL at the boundary, LIJK in HBM
 STREAM_SIMPLE, STREAM_ESOTWIST = 0, 1
-PREC_F32, PREC_F64, PREC_F16, PREC_BF16 = 0, 1, 2, 3
+PREC_F32, PREC_F64, PREC_F16, PREC_BF16, PREC_E3M12 = 0, 1, 2, 3, 4
 FIELD_Q, FIELD_F, FIELD_NU, FIELD_O = 0, 1, 2, 3
 GEOM_FIXED, GEOM_ROTATING_NON_UPDATING, GEOM_ROTATING = 0, 1, 2
 

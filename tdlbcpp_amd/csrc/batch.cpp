@@ -416,8 +416,9 @@ extern "C" {
 
 int tdlb_batch_create(int32_t precision, int32_t layout, int32_t streaming, int32_t device_id, tdlb_batch_t* out) {
     if (!out) return fail(TDLB_ERR_INVALID, "null argument");
-    if (precision != TDLB_PREC_F32 && precision != TDLB_PREC_F16 && precision != TDLB_PREC_BF16)
-        return fail(TDLB_ERR_UNSUPPORTED, "batches run the float-arithmetic precisions (fp32, fp16, bf16)");
+    if (precision != TDLB_PREC_F32 && precision != TDLB_PREC_F16 && precision != TDLB_PREC_BF16 &&
+        precision != TDLB_PREC_E3M12)
+        return fail(TDLB_ERR_UNSUPPORTED, "batches run the float-arithmetic precisions (fp32, fp16, bf16, e3m12)");
     if (layout != TDLB_LAYOUT_IJKL && layout != TDLB_LAYOUT_LIJK && layout != TDLB_LAYOUT_IJKL_SOA)
         return fail(TDLB_ERR_INVALID, "layout");
     if (streaming != TDLB_STREAM_SIMPLE && streaming != TDLB_STREAM_ESOTWIST) return fail(TDLB_ERR_INVALID, "streaming");

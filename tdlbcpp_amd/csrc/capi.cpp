@@ -18,7 +18,7 @@
 // other cell is zero in the reference even if the array still holds an old
 // value (forcing's "F = 0 where O == 0", Forcing.hpp:141-151).
 //
-// Reduced-precision storage (TDLB_PREC_F16 / TDLB_PREC_BF16): the lazy phase
+// Reduced-precision storage (TDLB_PREC_F16 / BF16 / E3M12): the lazy phase
 // keeps the populations as 16-bit scaled deviations k_d (f_d - w_d rho0) and
 // computes in float; the explicit Q is a float array M (the FP32 reference
 // layout), written by materialise and read by the first collision of a run.
@@ -372,7 +372,8 @@ int tdlb_device_count(int* count) {
 
 int tdlb_flow_calc_nu_re_m(tdlb_flow* f, int32_t outer_radius, int32_t precision) {
     if (!f) return fail(TDLB_ERR_INVALID, "null flow");
-    if (precision == TDLB_PREC_F32 || precision == TDLB_PREC_F16 || precision == TDLB_PREC_BF16) {
+    if (precision == TDLB_PREC_F32 || precision == TDLB_PREC_F16 || precision == TDLB_PREC_BF16 ||
+        precision == TDLB_PREC_E3M12) {
         const float reM = (float)f->re_m_nondimensional, uav = (float)f->uav;
         const float Re_m = reM * M_PI / 2.0;
         const float nu = uav * static_cast<float>(outer_radius) / Re_m;
@@ -396,7 +397,7 @@ int tdlb_cu_create(const tdlb_cu_params* p, const tdlb_flow* f, tdlb_cu_t* out) 
         return fail(TDLB_ERR_INVALID, "layout");
     if (p->streaming != TDLB_STREAM_SIMPLE && p->streaming != TDLB_STREAM_ESOTWIST)
         return fail(TDLB_ERR_INVALID, "streaming");
-    if (p->precision < TDLB_PREC_F32 || p->precision > TDLB_PREC_BF16)
+    if (p->precision < TDLB_PREC_F32 || p->precision > TDLB_PREC_E3M12)
         return fail(TDLB_ERR_UNSUPPORTED, "precision");
     if (f->collision != 0) return fail(TDLB_ERR_UNSUPPORTED, "only the EgglesSomers collision is implemented");
     const int64_t xg = p->x + 2, yg = p->y + 2, zg = p->z + 2;

@@ -10,7 +10,7 @@
 // those promotions out as (D) casts and keeps the reference's left-to-right order
 // of terms, so that results are identical to the reference.  With P = float the
 // same expressions run entirely in FP32.  All code that includes this header
-// is compiled with -ffp-contract=off; es_collide_fast, the float-only collision
+// is compiled with -ffp-contract=off; es_collide_dev, the float-only collision
 // of the 16-bit storage formats, alone turns contraction on for its body.
 //
 // One deliberate rewrite: the reference divides each of the 18 relaxed moments
@@ -219,14 +219,20 @@ TDLB_HD inline void es_collide(const T* m, T fx, T fy, T fz, T& nuc, const Colli
 }
 
 // Moments and collision of one cell in float only, for the 16-bit storage
-// formats, which carry no bit-for-bit obligation: the same linear maps as
-// es_moments / es_collide (M17 and M18 are not formed: their relaxed values are
-// zero), factored through the sums and differences of opposite populations,
-// with multiply-adds fused.  Each relaxed moment is divided by 24 correctly
-// rounded (Div24<float, float>), so the density is not biased.  f in,
-// populations out (p may alias f); nuc as in es_collide.
-TDLB_HD inline void es_collide_fast(const float* f, float fx, float fy, float fz, float& nuc,
-                                    const CollideParams<float>& cp, float* p) {
+// formats, which carry no bit-for-bit obligation.  It works on deviations from a
+// rest state B (B_d = rho_b w_d): f[d] = f_d - B_d in, p[d] = p_d - B_d out (p
+// may alias f).  B's moments are (rho_b, 0, ..., 0) and B relaxes to itself,
+// so the deviations obey the same linear maps as es_moments / es_collide with
+// the density moment replaced by rho - rho_b; the density rho itself enters
+// only the velocity and the LES factor.  Every rounding is then relative to
+// the deviations (|f - B| ~ 1e-3 .. 5e-2) rather than to the populations
+// (~0.04 .. 0.1), which keeps the mass of a cell to ~1e-10 per step where the
+// same arithmetic on whole populations drifted by 1e-8.  The maps are factored
+// through the sums and differences of the nine opposite pairs, M17/M18 are not
+// formed (their relaxed values are zero), multiply-adds are fused.  nuc as in
+// es_collide.
+TDLB_HD inline void es_collide_dev(const float* f, float rho_b, float fx, float fy, float fz, float& nuc,
+                                   const CollideParams<float>& cp, float* p) {
 #pragma clang fp contract(fast)
     // sums (s) and differences (a) of the nine opposite pairs
     const float sx = f[Q01] + f[Q02], ax = f[Q01] - f[Q02];
@@ -239,7 +245,7 @@ TDLB_HD inline void es_collide_fast(const float* f, float fx, float fy, float fz
     const float sxmz = f[Q15] + f[Q16], axmz = f[Q15] - f[Q16];  // +x-z / -x+z
     const float symz = f[Q17] + f[Q18], aymz = f[Q17] - f[Q18];  // +y-z / -y+z
 
-    const float m01 = ((sx + sy) + (sz + sxy)) + ((sxz + syz) + (sxmy + sxmz)) + symz;
+    const float m01 = ((sx + sy) + (sz + sxy)) + ((sxz + syz) + (sxmy + sxmz)) + symz;  // rho - rho_b
     const float m02 = ax + (axy + axmy) + (axz + axmz);
     const float m03 = ay + (axy - axmy) + (ayz + aymz);
     const float m04 = az + (axz - axmz) + (ayz - aymz);
@@ -257,13 +263,14 @@ TDLB_HD inline void es_collide_fast(const float* f, float fx, float fy, float fz
     const float m15 = (axz - axmz) + (ayz - aymz) - 2.0f * az;
     const float m16 = 3.0f * ((axmz - axz) + (ayz - aymz));
 
-    const float inv_rho = 1.0f / m01;
+    const float rho = rho_b + m01;
+    const float inv_rho = 1.0f / rho;
     const float ux = inv_rho * (m02 + 0.5f * fx);
     const float uy = inv_rho * (m03 + 0.5f * fy);
     const float uz = inv_rho * (m04 + 0.5f * fz);
     float b = cp.b0, c = cp.c0;
     if (cp.use_les) {
-        const float fct = 3.0f / (m01 * (1.0f + 6.0f * (nuc + cp.nu)));
+        const float fct = 3.0f / (rho * (1.0f + 6.0f * (nuc + cp.nu)));
         const float hfx = 0.5f * fx, hfy = 0.5f * fy, hfz = 0.5f * fz;
         const float dudx = fct * ((m02 + hfx * ux) - m05);
         const float dvdy = fct * ((m03 + hfy * uy) - m07);
@@ -281,15 +288,15 @@ TDLB_HD inline void es_collide_fast(const float* f, float fx, float fy, float fz
         c = 1.0f - nut6;
     }
     const float jx2 = 2.0f * m02 + fx, jy2 = 2.0f * m03 + fy, jz2 = 2.0f * m04 + fz;
-    using Q = Div24<float, float>;
-    const float a01 = Q::apply(m01);
-    const float a02 = Q::apply(m02 + fx), a03 = Q::apply(m03 + fy), a04 = Q::apply(m04 + fz);
-    const float a05 = Q::apply((jx2 * ux - m05 * c) * b), a06 = Q::apply((jx2 * uy - m06 * c) * b);
-    const float a07 = Q::apply((jy2 * uy - m07 * c) * b), a08 = Q::apply((jx2 * uz - m08 * c) * b);
-    const float a09 = Q::apply((jy2 * uz - m09 * c) * b), a10 = Q::apply((jz2 * uz - m10 * c) * b);
-    const float ng3 = -cp.g3;
-    const float a11 = Q::apply(ng3 * m11), a12 = Q::apply(ng3 * m12), a13 = Q::apply(ng3 * m13);
-    const float a14 = Q::apply(ng3 * m14), a15 = Q::apply(ng3 * m15), a16 = Q::apply(ng3 * m16);
+    // relaxed moments / 24; b / 24 and -g3 / 24 are folded into one factor each
+    const float b24 = b * (1.0f / 24.0f), g24 = cp.g3 * (-1.0f / 24.0f), r24 = 1.0f / 24.0f;
+    const float a01 = m01 * r24;
+    const float a02 = (m02 + fx) * r24, a03 = (m03 + fy) * r24, a04 = (m04 + fz) * r24;
+    const float a05 = (jx2 * ux - m05 * c) * b24, a06 = (jx2 * uy - m06 * c) * b24;
+    const float a07 = (jy2 * uy - m07 * c) * b24, a08 = (jx2 * uz - m08 * c) * b24;
+    const float a09 = (jy2 * uz - m09 * c) * b24, a10 = (jz2 * uz - m10 * c) * b24;
+    const float a11 = g24 * m11, a12 = g24 * m12, a13 = g24 * m13;
+    const float a14 = g24 * m14, a15 = g24 * m15, a16 = g24 * m16;
 
     // faces: p = C +- D
     const float two01 = 2.0f * a01;

@@ -198,7 +198,9 @@ struct tdlb_cu {
     // boundary (download, upload, checkpoint); they differ for TDLB_LAYOUT_IJKL_SOA
     int dlay() const { return prm.layout == TDLB_LAYOUT_IJKL ? LayoutIJKL : LayoutLIJK; }
     int blay() const { return prm.layout == TDLB_LAYOUT_LIJK ? LayoutLIJK : LayoutIJKL; }
-    bool shifted() const { return prm.precision == TDLB_PREC_F16 || prm.precision == TDLB_PREC_BF16; }
+    bool shifted() const {
+        return prm.precision == TDLB_PREC_F16 || prm.precision == TDLB_PREC_BF16 || prm.precision == TDLB_PREC_E3M12;
+    }
     int prec() const { return prm.precision; }            // kernels on the stored populations
     int eprec() const { return prm.precision == TDLB_PREC_F64 ? 1 : 0; }  // kernels on the explicit Q
     void* eq() const { return shifted() ? M : Q[cur]; }   // the explicit Q array

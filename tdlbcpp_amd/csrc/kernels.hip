@@ -15,6 +15,7 @@
 
 #include <type_traits>
 
+#include "codec16.h"
 #include "device.h"
 #include "es_collide.h"
 #include "kernels.h"
@@ -69,15 +70,19 @@ struct PrecF64 {
     using S = double; using T = double; using N = double; using D = double;
     static constexpr bool kShift = false;
 };
-// The 16-bit formats: enc rounds the scaled deviation to the stored type
-// (round to nearest even), dec widens it exactly.  Nu (>= 0, 1e-6 .. 1e-2 in a
-// stirred tank) is stored in the same type, scaled by 2^10 in fp16 so that it
-// stays above fp16's smallest normal (6.1e-5) down to 6e-8.
+// The 16-bit formats: enc rounds the scaled deviation s to the stored type
+// (round to nearest even), dec widens it exactly; enc_dev / dec_dev take the
+// unscaled deviation s / k of direction d (k = shift_scale(d), a power of two),
+// also exactly.  Nu (>= 0, 1e-6 .. 1e-2 in a stirred tank) is stored in 16
+// bits, scaled by 2^10 in fp16 so that it stays above fp16's smallest normal
+// (6.1e-5) down to 6e-8.
 struct PrecF16 {
     using S = _Float16; using T = float; using N = _Float16; using D = float;
     static constexpr bool kShift = true;
     __device__ static S enc(float s) { return (S)s; }
     __device__ static float dec(S x) { return (float)x; }
+    __device__ static S enc_dev(float v, int d) { return enc(v * shift_scale(d)); }
+    __device__ static float dec_dev(S x, int d) { return dec(x) * (1.0f / shift_scale(d)); }
     __host__ __device__ static N enc_nu(float v) { return (N)(v * 1024.0f); }
     __host__ __device__ static float dec_nu(N x) { return (float)x * (1.0f / 1024.0f); }
 };
@@ -86,8 +91,22 @@ struct PrecBF16 {
     static constexpr bool kShift = true;
     __device__ static S enc(float s) { return (S)s; }
     __device__ static float dec(S x) { return (float)x; }
+    __device__ static S enc_dev(float v, int d) { return enc(v * shift_scale(d)); }
+    __device__ static float dec_dev(S x, int d) { return dec(x) * (1.0f / shift_scale(d)); }
     __host__ __device__ static N enc_nu(float v) { return (N)v; }
     __host__ __device__ static float dec_nu(N x) { return (float)x; }
+};
+// E3M12 (codec16.h): 12 fraction bits for the deviations, the scale k folded
+// into the code's exponent; Nu as in PrecF16.
+struct PrecE3M12 {
+    using S = uint16_t; using T = float; using N = _Float16; using D = float;
+    static constexpr bool kShift = true;
+    __device__ static S enc(float s) { return e3m12_encode<0>(s); }
+    __device__ static float dec(S x) { return e3m12_decode<0>(x); }
+    __device__ static S enc_dev(float v, int d) { return d < 6 ? e3m12_encode<1>(v) : e3m12_encode<2>(v); }
+    __device__ static float dec_dev(S x, int d) { return d < 6 ? e3m12_decode<1>(x) : e3m12_decode<2>(x); }
+    __host__ __device__ static N enc_nu(float v) { return PrecF16::enc_nu(v); }
+    __host__ __device__ static float dec_nu(N x) { return PrecF16::dec_nu(x); }
 };
 // bit patterns moved unchanged
 template <typename W>
@@ -123,6 +142,20 @@ __device__ __forceinline__ void store_pop(typename PR::S* __restrict__ Q, uint64
     if constexpr (PR::kShift) Q[a] = PR::enc(fmaf(v, shift_scale(d), -shift_offset(rho0)));
     else Q[a] = v;
 }
+// The fused collision of the 16-bit formats works on deviations from the rest
+// populations c0 / k (es_collide_dev): f - c0 / k = dec(s) / k and the stored
+// value of a deviation p' is k p', both exact (k a power of two), so the
+// 16-bit rounding is the only one between collisions.
+template <typename PR>
+__device__ __forceinline__ float load_dev(const typename PR::S* __restrict__ Q, uint64_t a, int d) {
+    return PR::dec_dev(Q[a], d);
+}
+template <typename PR>
+__device__ __forceinline__ void store_dev(typename PR::S* __restrict__ Q, uint64_t a, int d, float v) {
+    Q[a] = PR::enc_dev(v, d);
+}
+// density of the rest populations c0 / k summed over the 18 directions
+__device__ __forceinline__ float rest_density(float rho0) { return 6.0f * shift_offset(rho0); }
 
 // ------------------------------------------------------------------ fills
 // every slot of the population array, row padding included
@@ -211,6 +244,7 @@ __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, ty
         const uint64_t a = qaddr<L>(g, home[d], EVEN ? d : opposite(d));
         // reduced-precision storage keeps the moments of MODE_COLL in mout
         if constexpr (PR::kShift && MODE == MODE_COLL) v[d] = mout[a];
+        else if constexpr (PR::kShift && MODE == MODE_FUSED) v[d] = load_dev<PR>(Q, a, d);
         else v[d] = load_pop<PR>(Q, a, d, rho0);
     }
 
@@ -231,7 +265,7 @@ __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, ty
     T nuc = P.use_les ? load_nu<PR>(Nu, n) : T(0);
     T p[kQ];
     if constexpr (MODE == MODE_FUSED && PR::kShift) {
-        es_collide_fast(v, fx, fy, fz, nuc, P, p);
+        es_collide_dev(v, rest_density(rho0), fx, fy, fz, nuc, P, p);
     } else if (MODE == MODE_FUSED) {
         T m[kQ];
         es_moments<T, D>(v, m);
@@ -250,7 +284,8 @@ __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, ty
     for (int d = 0; d < kQ; d++) {
         const int e = opposite(d);
         const uint32_t h = n2 + (uint32_t)((twist_dx(e) ? di2 : 0) + (twist_dy(e) ? dj2 : 0) + (twist_dz(e) ? dk2 : 0));
-        store_pop<PR>(Q, qaddr<L>(g, h, EVEN ? e : opposite(e)), d, p[d], rho0);
+        if constexpr (PR::kShift && MODE == MODE_FUSED) store_dev<PR>(Q, qaddr<L>(g, h, EVEN ? e : opposite(e)), d, p[d]);
+        else store_pop<PR>(Q, qaddr<L>(g, h, EVEN ? e : opposite(e)), d, p[d], rho0);
     }
 }
 
@@ -294,7 +329,9 @@ __device__ __forceinline__ void simple_cell(const typename PR::S* src,
             const bool ok = (ex > 0 ? im : ex < 0 ? ip : true) && (ey > 0 ? jm : ey < 0 ? jp : true) &&
                             (ez > 0 ? km : ez < 0 ? kp : true);
             const int32_t off = -(ex * (int32_t)g.qyz + ey * (int32_t)g.qz + ez);
-            v[d] = load_pop<PR>(src, qaddr<L>(g, ok ? qn + (uint32_t)off : qn, d), d, rho0);
+            const uint64_t a = qaddr<L>(g, ok ? qn + (uint32_t)off : qn, d);
+            if constexpr (PR::kShift && MODE == MODE_FUSED) v[d] = load_dev<PR>(src, a, d);
+            else v[d] = load_pop<PR>(src, a, d, rho0);
         }
     } else {
 #pragma unroll
@@ -321,7 +358,7 @@ __device__ __forceinline__ void simple_cell(const typename PR::S* src,
     T nuc = P.use_les ? load_nu<PR>(Nu, n) : T(0);
     T p[kQ];
     if constexpr (MODE == MODE_FUSED && PR::kShift) {
-        es_collide_fast(v, fx, fy, fz, nuc, P, p);
+        es_collide_dev(v, rest_density(rho0), fx, fy, fz, nuc, P, p);
     } else if (MODE == MODE_FUSED) {
         T m[kQ];
         es_moments<T, D>(v, m);
@@ -331,7 +368,10 @@ __device__ __forceinline__ void simple_cell(const typename PR::S* src,
     }
     if (P.use_les) store_nu<PR>(Nu, n, nuc);
 #pragma unroll
-    for (int d = 0; d < kQ; d++) store_pop<PR>(dst, qaddr<L>(g, qn, d), d, p[d], rho0);
+    for (int d = 0; d < kQ; d++) {
+        if constexpr (PR::kShift && MODE == MODE_FUSED) store_dev<PR>(dst, qaddr<L>(g, qn, d), d, p[d]);
+        else store_pop<PR>(dst, qaddr<L>(g, qn, d), d, p[d], rho0);
+    }
 }
 
 template <typename PR, int L, int MODE>
@@ -1215,14 +1255,16 @@ hipError_t force_level_t(int momsrc, int even, const PointsDev& p, int64_t begin
 
 }  // namespace
 
-// precision index -> traits: 0 fp32, 1 fp64, 2 fp16 storage, 3 bf16 storage
+// precision index -> traits: 0 fp32, 1 fp64, 2 fp16 storage, 3 bf16 storage,
+// 4 e3m12 storage (the C ABI validates the index at tdlb_cu_create)
 #define TDLB_WITH_PREC(prec, ...)                      \
     do {                                               \
         switch (prec) {                                \
             case 0: { using PR = PrecF32; __VA_ARGS__; } break;  \
             case 1: { using PR = PrecF64; __VA_ARGS__; } break;  \
             case 2: { using PR = PrecF16; __VA_ARGS__; } break;  \
-            default: { using PR = PrecBF16; __VA_ARGS__; } break; \
+            case 3: { using PR = PrecBF16; __VA_ARGS__; } break; \
+            default: { using PR = PrecE3M12; __VA_ARGS__; } break; \
         }                                              \
     } while (0)
 
@@ -1357,6 +1399,8 @@ hipError_t launch_ghost_fill_decode(int prec, int layout, const void* src, void*
         TDLB_MOVE(k_ghost_fill, PrecF16, ghost_count(g), (const PrecF16::S*)src, (float*)dst, g, rho0);
     else if (prec == 3)
         TDLB_MOVE(k_ghost_fill, PrecBF16, ghost_count(g), (const PrecBF16::S*)src, (float*)dst, g, rho0);
+    else if (prec == 4)
+        TDLB_MOVE(k_ghost_fill, PrecE3M12, ghost_count(g), (const PrecE3M12::S*)src, (float*)dst, g, rho0);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -1590,7 +1634,7 @@ hipError_t launch_regrid(int esize, int layout, const void* oldQ, void* newQ, in
 }
 
 namespace {
-// Nu between the 16-bit storage of prec 2/3 and the float array of the boundary
+// Nu between the 16-bit storage of prec 2/3/4 and the float array of the boundary
 template <typename PR, bool TO_FLOAT>
 __global__ void k_nu_convert(const void* __restrict__ src, void* __restrict__ dst, uint64_t n) {
     for (uint64_t a = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; a < n; a += (uint64_t)gridDim.x * blockDim.x) {
@@ -1608,6 +1652,9 @@ hipError_t launch_nu_convert(int prec, int to_float, const void* src, void* dst,
     } else if (prec == 3) {
         if (to_float) hipLaunchKernelGGL((k_nu_convert<PrecBF16, true>), dim3(nb), dim3(kBlock), 0, s, src, dst, n);
         else hipLaunchKernelGGL((k_nu_convert<PrecBF16, false>), dim3(nb), dim3(kBlock), 0, s, src, dst, n);
+    } else if (prec == 4) {
+        if (to_float) hipLaunchKernelGGL((k_nu_convert<PrecE3M12, true>), dim3(nb), dim3(kBlock), 0, s, src, dst, n);
+        else hipLaunchKernelGGL((k_nu_convert<PrecE3M12, false>), dim3(nb), dim3(kBlock), 0, s, src, dst, n);
     } else {
         return hipErrorInvalidValue;
     }
@@ -1915,6 +1962,7 @@ hipError_t with_batch_prec(int prec, int layout, int streaming, Fn&& fn) {
         case 0: return by_layout(PrecF32{});
         case 2: return by_layout(PrecF16{});
         case 3: return by_layout(PrecBF16{});
+        case 4: return by_layout(PrecE3M12{});
         default: return hipErrorInvalidValue;
     }
 }

@@ -69,7 +69,7 @@ def precision_report(T=96, steps=5000, precisions=(0, 2, 3), **kw):
     """Each precision against fp64: velocity L2 at the last step and of the velocity averaged over the
     last quarter of the run, power-number error, mass drift."""
     ref = tank_run(T, PREC_F64, steps, **kw)
-    names = {0: "fp32", 1: "fp64", 2: "fp16", 3: "bf16"}
+    names = {0: "fp32", 1: "fp64", 2: "fp16", 3: "bf16", 4: "e3m12"}
     out = {"grid": T, "steps": steps, "po_fp64": ref["po"], "mass_drift_fp64": ref["mass"][-1] / ref["mass0"] - 1.0,
            "variants": {}}
     for p in precisions:

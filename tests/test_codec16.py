@@ -1,0 +1,117 @@
+"""E3M12, the 16-bit code of TDLB_PREC_E3M12 (codec16.h), checked on the host.
+
+The format is defined by its value set: magnitude code q = 4096 e + j (e the
+3-bit exponent, j the 12-bit fraction) stands for 2^(e-9) (1 + j/4096) - 2^-9,
+bit 15 is the sign.  decode must give exactly that (checked against the closed
+form in double, which shares no code with the bit manipulation), encode must
+pick the nearest code with ties to even (checked by brute force against the
+neighbouring codes; within 2^-12 of a step from a midpoint the float sum
+|s| + C, rounded first, may tip it to the other neighbour), and the folded
+scales decode<L>/encode<L> must equal decode / 2^L and encode of v 2^L bit for
+bit.
+"""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+SRC = r"""
+#include <cmath>
+#include <cstdio>
+#include <cstdint>
+#include <limits>
+#include <random>
+#include "codec16.h"
+using namespace tdlb;
+static int fails = 0;
+#define CHECK(c) do { if (!(c)) { if (fails++ < 10) std::printf("FAIL line %d: %s\n", __LINE__, #c); } } while (0)
+static bool same(float a, float b) { return f32_bits(a) == f32_bits(b); }
+static double closed_form(uint32_t q) {
+    const int e = (int)(q >> 12), j = (int)(q & 4095u);
+    return std::ldexp(1.0 + j / 4096.0, e - 9) - std::ldexp(1.0, -9);
+}
+int main() {
+    // every magnitude code, both signs
+    double prev = -1.0;
+    for (uint32_t q = 0; q <= 0x7fff; q++) {
+        const float v = e3m12_decode<0>((uint16_t)q);
+        CHECK((double)v == closed_form(q));
+        CHECK((double)v > prev);
+        prev = v;
+        CHECK(e3m12_encode<0>(v) == q);
+        const float n = e3m12_decode<0>((uint16_t)(q | 0x8000u));
+        CHECK(same(n, -v));
+        CHECK(e3m12_encode<0>(n) == (q | 0x8000u));
+        // folded scales: exact division by 2 and 4
+        CHECK(same(e3m12_decode<1>((uint16_t)q), v * 0.5f));
+        CHECK(same(e3m12_decode<2>((uint16_t)q), v * 0.25f));
+        CHECK(same(e3m12_decode<2>((uint16_t)(q | 0x8000u)), n * 0.25f));
+        // the midpoint to the next code rounds to the even one of the two
+        if (q < 0x7fff) {
+            const double mid = 0.5 * ((double)v + closed_form(q + 1));
+            CHECK((double)(float)mid == mid);
+            const uint32_t c = e3m12_encode<0>((float)mid);
+            CHECK(c == ((q & 1u) ? q + 1 : q));
+        }
+    }
+    CHECK(e3m12_decode<0>(1) == std::ldexp(1.0f, -21));  // linear step below 2^-9
+    CHECK(e3m12_decode<0>(0) == 0.0f);
+    const double vmax = closed_form(0x7fff);
+    CHECK(vmax > 0.4979 && vmax < 0.4981);
+    // saturation
+    CHECK(e3m12_encode<0>(0.5f) == 0x7fff);
+    CHECK(e3m12_encode<0>(-3.0f) == 0xffff);
+    CHECK(e3m12_encode<0>(std::numeric_limits<float>::infinity()) == 0x7fff);
+    CHECK((e3m12_encode<0>(std::numeric_limits<float>::quiet_NaN()) & 0x7fffu) == 0x7fff);
+    // random values: nearest code by brute force against the neighbours
+    std::mt19937_64 rng(3);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    long tested = 0, near_mid = 0;
+    for (int t = 0; t < 2000000; t++) {
+        const double mag = t & 1 ? U(rng) * vmax : std::ldexp(1.0, -(int)(U(rng) * 30.0)) * U(rng) * 0.49;
+        const float s = (float)(t & 2 ? -mag : mag);
+        const uint32_t c = e3m12_encode<0>(s);
+        const uint32_t q = c & 0x7fffu;
+        CHECK(((c >> 15) & 1u) == (s < 0.0f ? 1u : 0u) || q == 0);
+        const double a = std::fabs((double)s);
+        const double err = std::fabs(closed_form(q) - a);
+        // the nearest code, except where a lies within 2^-12 of a step from the
+        // midpoint of two codes: the float sum |s| + C is rounded before the code
+        // is (double rounding), which may then pick the other neighbour
+        for (int nb = -1; nb <= 1; nb += 2) {
+            if ((nb < 0 && q == 0) || (nb > 0 && q == 0x7fff)) continue;
+            const double other = std::fabs(closed_form(q + nb) - a);
+            if (err > other) {
+                const double step = std::fabs(closed_form(q + nb) - closed_form(q));
+                CHECK(err - other <= 2.0 * std::ldexp(step, -12));
+                near_mid++;
+            }
+        }
+        // the folded encoders see v = s / k
+        CHECK(e3m12_encode<1>(s * 0.5f) == c);
+        CHECK(e3m12_encode<2>(s * 0.25f) == c);
+        tested++;
+    }
+    std::printf("%ld %ld %d\n", tested, near_mid, fails);
+    return fails != 0;
+}
+"""
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_e3m12_codec(tmp_path):
+    c = tmp_path / "codec.cpp"
+    c.write_text(SRC)
+    exe = tmp_path / "codec"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I",
+                    os.path.join(ROOT, "tdlbcpp_amd", "csrc"), "-o", str(exe), str(c)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
+    tested, near_mid, fails = map(int, r.stdout.split()[-3:])
+    assert fails == 0 and tested == 2000000
+    # double-rounding picks are rare (~1e-4 of the values) and bounded above
+    assert near_mid < tested // 1000

@@ -40,8 +40,9 @@ SPECS = [(36, 0.10), (48, 0.07), (40, 0.12), (36, 0.05)]
     (td.PREC_BF16, td.STREAM_SIMPLE, td.LAYOUT_LIJK, True, False),
     (td.PREC_F32, td.STREAM_ESOTWIST, td.LAYOUT_IJKL, True, True),
     (td.PREC_F16, td.STREAM_SIMPLE, td.LAYOUT_IJKL_SOA, True, True),
+    (td.PREC_E3M12, td.STREAM_ESOTWIST, td.LAYOUT_LIJK, True, True),
 ], ids=["fp16-Eso-LIJK-graph", "fp32-Simple-IJKL-nograph", "bf16-Simple-LIJK-notune", "fp32-Eso-IJKL-graph",
-        "fp16-Simple-IJKL_SOA-graph"])
+        "fp16-Simple-IJKL_SOA-graph", "e3m12-Eso-LIJK-graph"])
 def test_batch_members_equal_solo_runs(cfg):
     prec, st, ly, graph, tune = cfg
     case(f"batch vs solo {cfg}")

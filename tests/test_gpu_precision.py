@@ -77,7 +77,11 @@ def test_macroscopic_and_sums_match_oracle(layout, stream):
     assert red["torque"] != 0.0
 
 
-@pytest.mark.parametrize("prec", [td.PREC_F16, td.PREC_BF16], ids=["fp16", "bf16"])
+PREC16 = [td.PREC_F16, td.PREC_BF16, td.PREC_E3M12]
+PREC16_IDS = ["fp16", "bf16", "e3m12"]
+
+
+@pytest.mark.parametrize("prec", PREC16, ids=PREC16_IDS)
 def test_16bit_storage_rejects_per_call_operations(prec):
     case(f"16-bit per-call rejection {prec}")
     _, gflow = flow_for_tank(36)
@@ -91,7 +95,7 @@ def test_16bit_storage_rejects_per_call_operations(prec):
 
 
 @pytest.mark.parametrize("stream", STREAMS, ids=["Simple", "Esotwist"])
-@pytest.mark.parametrize("prec", [td.PREC_F16, td.PREC_BF16], ids=["fp16", "bf16"])
+@pytest.mark.parametrize("prec", PREC16, ids=PREC16_IDS)
 def test_16bit_storage_tracks_fp64(prec, stream):
     """A short tank run in 16-bit storage stays close to fp64 and conserves mass."""
     T, steps = 48, 300
@@ -110,10 +114,20 @@ def test_16bit_storage_tracks_fp64(prec, stream):
     print(f"[gpu case] velocity L2 vs fp64: {l2:.3e} (fp32 {l2_32:.3e}); Po {r['po']:.5f} vs {ref['po']:.5f} "
           f"(rel {po_err:.2e}); mass drift {drift:.3e} (fp64 {drift64:.3e})", flush=True)
     assert np.isfinite(r["mac"]).all()
-    fp16 = prec == td.PREC_F16
-    assert l2 <= (2e-3 if fp16 else 5e-2)
-    assert po_err <= (1e-2 if fp16 else 5e-2)
-    assert abs(drift - drift64) <= (1e-4 if fp16 else 5e-3)
+    # Simple streaming is unstable on this tank (fp32 already differs from fp64
+    # by ~7e-4 after 300 steps, Esotwist by ~2e-5), so its L2 measures how much
+    # each format seeds the instability: E3M12's linear step below 2^-9 (5e-7)
+    # is coarser than fp16's at the tiny deviations of the fluid near rest.
+    esotwist = stream[0] == td.STREAM_ESOTWIST
+    l2_max, po_max, drift_max = {td.PREC_F16: (2e-3, 1e-2, 1e-4), td.PREC_BF16: (5e-2, 5e-2, 5e-3),
+                                 td.PREC_E3M12: (1e-3 if esotwist else 1e-2, 5e-3 if esotwist else 1e-2,
+                                                 1e-4)}[prec]
+    assert l2 <= l2_max
+    assert po_err <= po_max
+    assert abs(drift - drift64) <= drift_max
+    if esotwist:
+        # the deviation-form collision conserves the mass of the 16-bit formats
+        assert abs(drift) <= 1e-6
 
 
 def test_fp64_rest_tank_conserves_mass_exactly():
