@@ -44,24 +44,30 @@ TDLB_HD constexpr uint32_t e3m12_base() {
     return (uint32_t)(127 + kE3M12OffsetExp - L2K) << 23;
 }
 
+// Codes are int16_t.  The decoder moves the sign bit straight to bit 31 and
+// the encoder builds the code in the upper half of a 32-bit word (stored from
+// there), a handful of integer operations each way.
 template <int L2K>
-TDLB_HD inline float e3m12_decode(uint16_t h) {
+TDLB_HD inline float e3m12_decode(int16_t h) {
     constexpr uint32_t base = e3m12_base<L2K>();
-    const float y = f32_from_bits(((uint32_t)(h & kE3M12MaxMag) << (23 - kE3M12Frac)) + base);
+    const uint32_t w = (uint16_t)h;
+    const float y = f32_from_bits(((w & kE3M12MaxMag) << (23 - kE3M12Frac)) + base);
     const float m = y - f32_from_bits(base);  // exact: y and C / k share the grid of C / k 2^-23
-    return f32_from_bits(f32_bits(m) | ((uint32_t)(h & 0x8000u) << 16));
+    return f32_from_bits(((w << 16) & 0x80000000u) | f32_bits(m));
 }
 
 template <int L2K>
-TDLB_HD inline uint16_t e3m12_encode(float v) {
+TDLB_HD inline int16_t e3m12_encode(float v) {
     constexpr uint32_t base = e3m12_base<L2K>();
     constexpr int drop = 23 - kE3M12Frac;
+    constexpr uint32_t rmax = ((uint32_t)kE3M12MaxMag << drop) | ((1u << drop) - 1u);
     const uint32_t yb = f32_bits(fabsf(v) + f32_from_bits(base));
     // round to nearest-even at bit `drop` of bits(y) - bits(C / k) (bits(C / k) has no bits below 23)
-    const uint32_t r = yb - base + ((1u << (drop - 1)) - 1u) + ((yb >> drop) & 1u);
-    uint32_t q = r >> drop;
-    q = q > kE3M12MaxMag ? kE3M12MaxMag : q;
-    return (uint16_t)(q | ((f32_bits(v) >> 16) & 0x8000u));
+    uint32_t r = yb - base + ((1u << (drop - 1)) - 1u) + ((yb >> drop) & 1u);
+    r = r > rmax ? rmax : r;  // saturate: magnitude code at most kE3M12MaxMag
+    // magnitude code to bits 16..30, the sign of v to bit 31; the code is the upper half
+    const uint32_t w = (r << (16 - drop)) | (f32_bits(v) & 0x80000000u);
+    return (int16_t)(w >> 16);
 }
 
 }  // namespace tdlb

@@ -230,7 +230,14 @@ TDLB_HD inline void es_collide(const T* m, T fx, T fy, T fz, T& nuc, const Colli
 // same arithmetic on whole populations drifted by 1e-8.  The maps are factored
 // through the sums and differences of the nine opposite pairs, M17/M18 are not
 // formed (their relaxed values are zero), multiply-adds are fused.  nuc as in
-// es_collide.
+// es_collide.  Ops supplies the three reciprocals and the square root of the
+// LES: exactly rounded by default (ExactOps); a device caller may pass the
+// hardware's 1-ulp instructions, whose error is far below the 16-bit rounding.
+struct ExactOps {
+    TDLB_HD static float rcp(float x) { return 1.0f / x; }
+    TDLB_HD static float sqrt(float x) { return sqrtf(x); }
+};
+template <typename Ops = ExactOps>
 TDLB_HD inline void es_collide_dev(const float* f, float rho_b, float fx, float fy, float fz, float& nuc,
                                    const CollideParams<float>& cp, float* p) {
 #pragma clang fp contract(fast)
@@ -264,13 +271,13 @@ TDLB_HD inline void es_collide_dev(const float* f, float rho_b, float fx, float 
     const float m16 = 3.0f * ((axmz - axz) + (ayz - aymz));
 
     const float rho = rho_b + m01;
-    const float inv_rho = 1.0f / rho;
+    const float inv_rho = Ops::rcp(rho);
     const float ux = inv_rho * (m02 + 0.5f * fx);
     const float uy = inv_rho * (m03 + 0.5f * fy);
     const float uz = inv_rho * (m04 + 0.5f * fz);
     float b = cp.b0, c = cp.c0;
     if (cp.use_les) {
-        const float fct = 3.0f / (rho * (1.0f + 6.0f * (nuc + cp.nu)));
+        const float fct = 3.0f * Ops::rcp(rho * (1.0f + 6.0f * (nuc + cp.nu)));
         const float hfx = 0.5f * fx, hfy = 0.5f * fy, hfz = 0.5f * fz;
         const float dudx = fct * ((m02 + hfx * ux) - m05);
         const float dvdy = fct * ((m03 + hfy * uy) - m07);
@@ -282,9 +289,9 @@ TDLB_HD inline void es_collide_dev(const float* f, float rho_b, float fx, float 
         const float sh_yz = fct2 * ((m04 + hfz * uy) - m09);
         const float sh = 2.0f * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + sh_xy * sh_xy + sh_xz * sh_xz +
                          sh_yz * sh_yz - (2.0f / 3.0f) * divv * divv;
-        nuc = cp.cs0 * cp.cs0 * sqrt_abs(sh);
+        nuc = cp.cs0 * cp.cs0 * Ops::sqrt(fabsf(sh));
         const float nut6 = 6.0f * (nuc + cp.nu);
-        b = 1.0f / (1.0f + nut6);
+        b = Ops::rcp(1.0f + nut6);
         c = 1.0f - nut6;
     }
     const float jx2 = 2.0f * m02 + fx, jy2 = 2.0f * m03 + fy, jz2 = 2.0f * m04 + fz;

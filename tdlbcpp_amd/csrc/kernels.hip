@@ -99,7 +99,7 @@ struct PrecBF16 {
 // E3M12 (codec16.h): 12 fraction bits for the deviations, the scale k folded
 // into the code's exponent; Nu as in PrecF16.
 struct PrecE3M12 {
-    using S = uint16_t; using T = float; using N = _Float16; using D = float;
+    using S = int16_t; using T = float; using N = _Float16; using D = float;
     static constexpr bool kShift = true;
     __device__ static S enc(float s) { return e3m12_encode<0>(s); }
     __device__ static float dec(S x) { return e3m12_decode<0>(x); }
@@ -156,6 +156,12 @@ __device__ __forceinline__ void store_dev(typename PR::S* __restrict__ Q, uint64
 }
 // density of the rest populations c0 / k summed over the 18 directions
 __device__ __forceinline__ float rest_density(float rho0) { return 6.0f * shift_offset(rho0); }
+// the hardware's reciprocal and square root (1 ulp) for the float collision of
+// the 16-bit formats; the IEEE-exact forms expand to ~9 instructions each
+struct HwOps {
+    __device__ static float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    __device__ static float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+};
 
 // ------------------------------------------------------------------ fills
 // every slot of the population array, row padding included
@@ -265,7 +271,7 @@ __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, ty
     T nuc = P.use_les ? load_nu<PR>(Nu, n) : T(0);
     T p[kQ];
     if constexpr (MODE == MODE_FUSED && PR::kShift) {
-        es_collide_dev(v, rest_density(rho0), fx, fy, fz, nuc, P, p);
+        es_collide_dev<HwOps>(v, rest_density(rho0), fx, fy, fz, nuc, P, p);
     } else if (MODE == MODE_FUSED) {
         T m[kQ];
         es_moments<T, D>(v, m);
@@ -358,7 +364,7 @@ __device__ __forceinline__ void simple_cell(const typename PR::S* src,
     T nuc = P.use_les ? load_nu<PR>(Nu, n) : T(0);
     T p[kQ];
     if constexpr (MODE == MODE_FUSED && PR::kShift) {
-        es_collide_dev(v, rest_density(rho0), fx, fy, fz, nuc, P, p);
+        es_collide_dev<HwOps>(v, rest_density(rho0), fx, fy, fz, nuc, P, p);
     } else if (MODE == MODE_FUSED) {
         T m[kQ];
         es_moments<T, D>(v, m);

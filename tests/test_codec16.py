@@ -29,6 +29,9 @@ using namespace tdlb;
 static int fails = 0;
 #define CHECK(c) do { if (!(c)) { if (fails++ < 10) std::printf("FAIL line %d: %s\n", __LINE__, #c); } } while (0)
 static bool same(float a, float b) { return f32_bits(a) == f32_bits(b); }
+// codes as unsigned 16-bit patterns
+template <int L> static uint32_t enc(float v) { return (uint16_t)e3m12_encode<L>(v); }
+template <int L> static float dec(uint32_t c) { return e3m12_decode<L>((int16_t)(uint16_t)c); }
 static double closed_form(uint32_t q) {
     const int e = (int)(q >> 12), j = (int)(q & 4095u);
     return std::ldexp(1.0 + j / 4096.0, e - 9) - std::ldexp(1.0, -9);
@@ -37,35 +40,35 @@ int main() {
     // every magnitude code, both signs
     double prev = -1.0;
     for (uint32_t q = 0; q <= 0x7fff; q++) {
-        const float v = e3m12_decode<0>((uint16_t)q);
+        const float v = dec<0>((uint32_t)q);
         CHECK((double)v == closed_form(q));
         CHECK((double)v > prev);
         prev = v;
-        CHECK(e3m12_encode<0>(v) == q);
-        const float n = e3m12_decode<0>((uint16_t)(q | 0x8000u));
+        CHECK(enc<0>(v) == q);
+        const float n = dec<0>((uint32_t)(q | 0x8000u));
         CHECK(same(n, -v));
-        CHECK(e3m12_encode<0>(n) == (q | 0x8000u));
+        CHECK(enc<0>(n) == (q | 0x8000u));
         // folded scales: exact division by 2 and 4
-        CHECK(same(e3m12_decode<1>((uint16_t)q), v * 0.5f));
-        CHECK(same(e3m12_decode<2>((uint16_t)q), v * 0.25f));
-        CHECK(same(e3m12_decode<2>((uint16_t)(q | 0x8000u)), n * 0.25f));
+        CHECK(same(dec<1>((uint32_t)q), v * 0.5f));
+        CHECK(same(dec<2>((uint32_t)q), v * 0.25f));
+        CHECK(same(dec<2>((uint32_t)(q | 0x8000u)), n * 0.25f));
         // the midpoint to the next code rounds to the even one of the two
         if (q < 0x7fff) {
             const double mid = 0.5 * ((double)v + closed_form(q + 1));
             CHECK((double)(float)mid == mid);
-            const uint32_t c = e3m12_encode<0>((float)mid);
+            const uint32_t c = enc<0>((float)mid);
             CHECK(c == ((q & 1u) ? q + 1 : q));
         }
     }
-    CHECK(e3m12_decode<0>(1) == std::ldexp(1.0f, -21));  // linear step below 2^-9
-    CHECK(e3m12_decode<0>(0) == 0.0f);
+    CHECK(dec<0>(1) == std::ldexp(1.0f, -21));  // linear step below 2^-9
+    CHECK(dec<0>(0) == 0.0f);
     const double vmax = closed_form(0x7fff);
     CHECK(vmax > 0.4979 && vmax < 0.4981);
     // saturation
-    CHECK(e3m12_encode<0>(0.5f) == 0x7fff);
-    CHECK(e3m12_encode<0>(-3.0f) == 0xffff);
-    CHECK(e3m12_encode<0>(std::numeric_limits<float>::infinity()) == 0x7fff);
-    CHECK((e3m12_encode<0>(std::numeric_limits<float>::quiet_NaN()) & 0x7fffu) == 0x7fff);
+    CHECK(enc<0>(0.5f) == 0x7fff);
+    CHECK(enc<0>(-3.0f) == 0xffff);
+    CHECK(enc<0>(std::numeric_limits<float>::infinity()) == 0x7fff);
+    CHECK((enc<0>(std::numeric_limits<float>::quiet_NaN()) & 0x7fffu) == 0x7fff);
     // random values: nearest code by brute force against the neighbours
     std::mt19937_64 rng(3);
     std::uniform_real_distribution<double> U(0.0, 1.0);
@@ -73,7 +76,7 @@ int main() {
     for (int t = 0; t < 2000000; t++) {
         const double mag = t & 1 ? U(rng) * vmax : std::ldexp(1.0, -(int)(U(rng) * 30.0)) * U(rng) * 0.49;
         const float s = (float)(t & 2 ? -mag : mag);
-        const uint32_t c = e3m12_encode<0>(s);
+        const uint32_t c = enc<0>(s);
         const uint32_t q = c & 0x7fffu;
         CHECK(((c >> 15) & 1u) == (s < 0.0f ? 1u : 0u) || q == 0);
         const double a = std::fabs((double)s);
@@ -91,8 +94,8 @@ int main() {
             }
         }
         // the folded encoders see v = s / k
-        CHECK(e3m12_encode<1>(s * 0.5f) == c);
-        CHECK(e3m12_encode<2>(s * 0.25f) == c);
+        CHECK(enc<1>(s * 0.5f) == c);
+        CHECK(enc<2>(s * 0.25f) == c);
         tested++;
     }
     std::printf("%ld %ld %d\n", tested, near_mid, fails);
