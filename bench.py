@@ -101,31 +101,63 @@ def run_variant(td, hip, grid, streaming, layout, steps, warmup, precision=0, re
     return res
 
 
-def cpu_baseline(sample_grid=96, max_seconds=20.0):
-    """The oracle (single-threaded CPU restatement of the reference path) on the same
-    tank at sample_grid^3, Simple streaming, IJKL, FP32, LES: the configuration
-    main.cpp runs on the CPU."""
-    import numpy as np
+def host_cpu():
+    """CPU model, logical CPUs of the host and CPUs this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"cpu": model, "nproc": os.cpu_count(), "cpus_available": avail}
+
+
+def cpu_baseline(sample_grid=96, max_seconds=20.0, omp_seconds=8.0, omp_threads=16):
+    """The oracle (CPU restatement of the reference path) on the same tank at
+    sample_grid^3, Simple streaming, IJKL, FP32, LES: the configuration main.cpp
+    runs on the CPU.  The baseline is one host core, as the reference is
+    single-threaded; a second leg runs the oracle's collision and moments on
+    OpenMP threads (push, bounce-back and forcing stay serial), labelled apart."""
     import tdlbcpp_amd as td
     from oracle import oracle as O
     from tests.helpers import ReferenceLoop, flow_for_tank
     oflow, _ = flow_for_tank(sample_grid, re=7300.0)
-    o = O.OracleCU(sample_grid, dtype=O.F32, layout=O.IJKL, streaming=O.SIMPLE, use_les=True, **oflow)
     rp = td.rushton_defaults(sample_grid)
-    o.initialise(1.0)
-    loop = ReferenceLoop(o, rp)
-    loop.run(1)  # warm caches
-    steps, t0 = 0, time.time()
-    while True:
-        loop.run(1)
-        steps += 1
-        el = time.time() - t0
-        if el > max_seconds or steps >= 500:
-            break
-    del np
-    return dict(value=round(sample_grid ** 3 * steps / el / 1e6, 3), unit="MLUPS", cores=1, kind="port",
-                sample=f"oracle/liboracle.so: Rushton tank {sample_grid}^3, Simple/IJKL, FP32, LES, "
-                       f"{steps} main-loop steps in {el:.1f} s on one host core")
+
+    def timed(threads, seconds):
+        O.set_threads(threads)
+        o = O.OracleCU(sample_grid, dtype=O.F32, layout=O.IJKL, streaming=O.SIMPLE, use_les=True, **oflow)
+        o.initialise(1.0)
+        loop = ReferenceLoop(o, rp)
+        loop.run(1)  # warm caches
+        steps, t0 = 0, time.time()
+        while True:
+            loop.run(1)
+            steps += 1
+            el = time.time() - t0
+            if el > seconds or steps >= 500:
+                break
+        return sample_grid ** 3 * steps / el / 1e6, steps, el
+
+    host = host_cpu()
+    try:
+        mlups, steps, el = timed(1, max_seconds)
+        out = dict(value=round(mlups, 3), unit="MLUPS", cores=1, kind="port",
+                   sample=f"oracle/liboracle.so: Rushton tank {sample_grid}^3, Simple/IJKL, FP32, LES, "
+                          f"{steps} main-loop steps in {el:.1f} s on one host core", host=host)
+        threads = min(omp_threads, host["cpus_available"] or 1)
+        if threads > 1 and omp_seconds > 0:
+            mlups, steps, el = timed(threads, omp_seconds)
+            out["openmp"] = dict(value=round(mlups, 3), unit="MLUPS", cores=threads,
+                                 sample=f"same tank, collision and moments on {threads} OpenMP threads, push, "
+                                        f"bounce-back and forcing serial: {steps} steps in {el:.1f} s")
+    finally:
+        O.set_threads(1)
+    return out
 
 
 def batch_tanks(n_tanks=64, steps=2000, seed=1):
@@ -294,6 +326,8 @@ def main():
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--variants", default="all", help="all, precision (fp32 + fp16/bf16/e3m12 Esotwist/LIJK) or headline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-omp-seconds", type=float, default=8.0,
+                    help="seconds of the OpenMP leg of the CPU baseline (0 skips it)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="seconds of oracle main-loop steps timed for cpu_baseline")
     ap.add_argument("--workload", default="tank", choices=["tank", "batch", "serve", "accuracy"],
@@ -418,7 +452,7 @@ def main():
     out["roofline"]["copy_gbs"] = round(copy, 1)
     out["roofline"]["frac_of_copy"] = round(achieved / copy, 4)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(max_seconds=args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(max_seconds=args.cpu_seconds, omp_seconds=args.cpu_omp_seconds)
     print(json.dumps(out), flush=True)
     return 0
 

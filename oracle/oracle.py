@@ -33,6 +33,13 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def set_threads(n):
+    """OpenMP threads for collision and moments (1, the default, is the reference's
+    single-threaded path; results are the same for any count)."""
+    lib().orc_set_threads(int(n))
+    return lib().orc_get_threads()
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -48,6 +55,10 @@ def lib():
             getattr(L, name).argtypes = [vp]
             getattr(L, name).restype = None
         L.orc_initialise.argtypes = [vp, d]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_threads.restype = None
+        L.orc_get_threads.restype = ctypes.c_int
+        L.orc_set_threads(1)  # the reference's path is single-threaded
         L.orc_forcing.argtypes = [vp, vp, i64, d, d]
         L.orc_field_bytes.argtypes = [vp, ctypes.c_int]
         L.orc_field_bytes.restype = i64

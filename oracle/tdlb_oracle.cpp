@@ -2,8 +2,10 @@
 //
 // TEST INFRASTRUCTURE ONLY.  Nothing in the product links or calls this file:
 // it is loaded by tests/, by __graft_entry__.smoke() as the checker and by
-// bench.py's cpu_baseline leg.  It restates, in plain single-threaded C++, the
-// algorithm of TurbulentDynamics/tdLBCpp's CPU ComputeUnit:
+// bench.py's cpu_baseline leg.  It restates, in plain C++, the algorithm of
+// TurbulentDynamics/tdLBCpp's CPU ComputeUnit, single-threaded like the
+// reference unless orc_set_threads asks for more (then collision and moments,
+// whose cells are independent, run on OpenMP threads; every result is the same):
 //
 //   indexing / layouts     tdlbcpp/src/ComputeUnit.hpp:25-40,453-476; Field.hpp:18-144
 //   initialise             tdlbcpp/src/ComputeUnit.hpp:174-190; QVec.hpp:146-151
@@ -34,6 +36,7 @@
 #include <type_traits>
 #include <vector>
 #include <algorithm>
+#include <omp.h>
 
 namespace orc {
 
@@ -210,11 +213,14 @@ struct CU : Base {
                - 1.5 * q[Q10] - 1.5 * q[Q15] + 1.5 * q[Q11] + 1.5 * q[Q18] + 1.5 * q[Q12] + 1.5 * q[Q17];
     }
 
+    // cells are independent (each reads and writes only its own slots), so the
+    // threaded build splits the i-planes; results do not depend on the split
     void moments() override {
-        T q[18], m[18];
+#pragma omp parallel for schedule(static)
         for (int64_t i = 1; i <= xg1; i++)
             for (int64_t j = 1; j <= yg1; j++)
                 for (int64_t k = 1; k <= zg1; k++) {
+                    T q[18], m[18];
                     readCell(i, j, k, q);
                     momentsOf(q, m);
                     writeMomentsCell(i, j, k, m);
@@ -303,10 +309,11 @@ struct CU : Base {
     void collision() override {
         T b = 1.0 / (1.0 + 6 * flow.nu);
         T c = 1.0 - 6 * flow.nu;
-        T m[18];
+#pragma omp parallel for schedule(static)
         for (int64_t i = 1; i <= xg1; i++)
             for (int64_t j = 1; j <= yg1; j++)
                 for (int64_t k = 1; k <= zg1; k++) {
+                    T m[18];
                     int64_t n = index(i, j, k);
                     T f[3] = {F[3 * n + 0], F[3 * n + 1], F[3 * n + 2]};
                     readCell(i, j, k, m);
@@ -652,6 +659,9 @@ void* orc_create(int dtype, int layout, int streaming, int64_t x, int64_t y, int
     return new TagCU<LIJK, ESOTWIST>(x, y, z, ghost, f);
 }
 void orc_destroy(void* h) { delete static_cast<orc::Base*>(h); }
+// threads for collision and moments (1 = the reference's single-threaded path)
+void orc_set_threads(int n) { omp_set_num_threads(n < 1 ? 1 : n); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
 void orc_initialise(void* h, double rho) { static_cast<orc::Base*>(h)->initialise(rho); }
 void orc_collision(void* h) { static_cast<orc::Base*>(h)->collision(); }
 void orc_moments(void* h) { static_cast<orc::Base*>(h)->moments(); }

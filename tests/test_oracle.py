@@ -484,3 +484,27 @@ def test_vorticity_of_a_shear_flow_is_log_of_shear_squared():
     assert np.isnan(o.vorticity(0, 3, ex)[1, 3])
     # plane k = 4: same shear, same value away from the ghost layers
     np.testing.assert_allclose(o.vorticity(1, 4)[1:-1, 1:-1], np.log(a * a), rtol=1e-12)
+
+
+@pytest.mark.parametrize("streaming", [O.SIMPLE, O.ESOTWIST], ids=["Simple", "Esotwist"])
+def test_threaded_oracle_is_identical(streaming):
+    """The threaded CPU baseline (OpenMP collision and moments, cells split over
+    i-planes) gives the single-threaded oracle's fields bit for bit."""
+    import tdlbcpp_amd as td
+    from tests.helpers import ReferenceLoop, flow_for_tank
+    T = 36
+    out = []
+    try:
+        for threads in (1, 4):
+            assert O.set_threads(threads) == threads
+            oflow, _ = flow_for_tank(T, re=7300.0)
+            o = O.OracleCU(T, dtype=O.F32, layout=O.IJKL, streaming=streaming, use_les=True, **oflow)
+            o.initialise(1.0)
+            loop = ReferenceLoop(o, td.rushton_defaults(T))
+            loop.run(6)
+            out.append([o.get(f).copy() for f in (O.FIELD_Q, O.FIELD_F, O.FIELD_NU)])
+    finally:
+        O.set_threads(1)
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
+    assert np.abs(out[0][1]).max() > 0  # the forcing ran
