@@ -8,13 +8,16 @@ rest (rho = 1).  One step is one iteration of the reference main loop
 (tdlbcpp/src/main.cpp:336-455): blade rotation, collision, bounce-back,
 streaming, moments and IB forcing, all on the device.
 
-Every streaming x layout variant is run; the headline value is Esotwist/LIJK.
+Every streaming x layout variant is run, and Esotwist/LIJK also with 16-bit
+population storage (fp16, bf16, e3m12); the headline value is FP32 Esotwist/LIJK.
 value = interior cells x steps / seconds / 1e6 (MLUPS), timed with HIP events
 on the ComputeUnit's stream around exactly --steps steps after --warmup steps.
 The roofline entry uses the stream-collide kernel's own event-timed average
 duration and its algorithmic bytes (2 x 18 x 4 B populations + 2 x 4 B Nu per
-cell).  cpu_baseline times the CPU oracle (a single-threaded restatement of the
-reference's CPU path) on a bounded sample: the same tank at 96^3 for a few steps.
+cell).  cpu_baseline times the CPU oracle (a restatement of the reference's
+single-threaded CPU path) on one host core on a bounded sample, the same tank
+at 96^3 for ~15 s, and reports beside it, labelled, the oracle with collision
+and moments on OpenMP threads, and the host CPU.
 """
 import argparse
 import ctypes
