@@ -62,6 +62,42 @@ def test_checkpoint_write_read_round_trip(tmp_path, streaming, layout):
     np.testing.assert_array_equal(b.get(td.FIELD_Q), a.get(td.FIELD_Q))
 
 
+@pytest.mark.parametrize("prec", [td.PREC_F16, td.PREC_BF16, td.PREC_E3M12], ids=["fp16", "bf16", "e3m12"])
+def test_16bit_checkpoint_round_trip(tmp_path, prec):
+    """A 16-bit handle writes the reference's float files (moments decoded from the
+    16-bit populations, Nu widened), reads them back exactly, and its moments
+    agree with an fp32 tank run alongside to the 16-bit rounding."""
+    case(f"16-bit checkpoint round trip prec={prec}")
+    T, steps = 36, 7
+    flow = td.flow_for_tank(T)
+    rp = td.rushton_defaults(T)
+    ref = td.ComputeUnit(T, layout=td.LAYOUT_LIJK, streaming=td.STREAM_ESOTWIST, precision=td.PREC_F32, **flow)
+    a = td.ComputeUnit(T, layout=td.LAYOUT_LIJK, streaming=td.STREAM_ESOTWIST, precision=prec, **flow)
+    for cu in (ref, a):
+        cu.initialise(1.0)
+        cu.bind_rushton(rp)
+        cu.run(steps)
+    rc = td.lib().tdlb_cu_checkpoint_write(a._h, str(tmp_path).encode(), b"device", b'{"x": 1}')
+    assert rc == 0, td.lib().tdlb_last_error()
+    Q, F, Nu = a.get(td.FIELD_Q), a.get(td.FIELD_F), a.get(td.FIELD_NU)
+    assert Q.dtype == np.float32 and Nu.dtype == np.float32
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "checkpoint_grid.0.0.0.device.Q", dtype=np.float32), Q)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "checkpoint_grid.0.0.0.device.Nu", dtype=np.float32), Nu)
+    b = td.ComputeUnit(T, layout=td.LAYOUT_LIJK, streaming=td.STREAM_ESOTWIST, precision=prec, **flow)
+    missing = td.ctypes.c_int(-1)
+    assert td.lib().tdlb_cu_checkpoint_read(b._h, str(tmp_path).encode(), b"device", td.ctypes.byref(missing)) == 0
+    assert missing.value == 0
+    for f, want in ((td.FIELD_Q, Q), (td.FIELD_F, F), (td.FIELD_NU, Nu)):
+        np.testing.assert_array_equal(b.get(f), want)
+    # moments (rho, j, and the rest) of the 16-bit run against fp32: the 16-bit
+    # rounding of the deviations (relative step 2^-11 fp16, 2^-8 bf16, 2^-12 e3m12)
+    q16 = Q.reshape(18, -1)
+    q32 = ref.get(td.FIELD_Q).reshape(18, -1)
+    tol = {td.PREC_F16: 2e-5, td.PREC_BF16: 2e-4, td.PREC_E3M12: 1e-5}[prec]
+    np.testing.assert_allclose(q16, q32, rtol=0, atol=tol)
+    assert np.abs(q32[1:4]).max() > 1e-4  # the impeller has set the fluid moving
+
+
 def test_checkpoint_read_reports_missing_and_wrong_size(tmp_path):
     case("checkpoint missing / wrong size")
     a = td.ComputeUnit(8, layout=td.LAYOUT_LIJK, streaming=td.STREAM_SIMPLE)
