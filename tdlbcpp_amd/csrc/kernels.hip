@@ -115,13 +115,15 @@ struct RawBits {
     static constexpr bool kShift = false;
 };
 
-// Cache hints of the stream-collide accesses.  A wavefront of the fused
+// Cache hints of the stream-collide accesses.  In LIJK a wavefront of the fused
 // Esotwist kernel reads and writes most slot planes at its own k positions,
 // lines that no other wavefront touches: those accesses carry the
 // non-temporal hint (nt).  The planes it reads at k+1 share their edge line
 // with the next wavefront, which finds it in the cache only without the hint.
 // Measured at 512^3 (FP32): hint on the unshifted planes 4.4 % faster than no
-// hint; on loads only 1.8 % slower; on every access 6 % slower.  The hint is a
+// hint; on loads only 1.8 % slower; on every access 6 % slower; in IJKL (a
+// line holds slots of several cells) 35 % slower, so IJKL takes no hint.  The
+// Simple pull showed no difference either way and takes none.  The hint is a
 // template argument: each instantiation holds one kind of access, and call sites
 // pick the instantiation per plane (a runtime flag lets the compiler merge the
 // two accesses of one address and drop the hint).
@@ -273,13 +275,14 @@ __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, ty
     for (int d = 0; d < kQ; d++) {
         const uint64_t a = qaddr<L>(g, home[d], EVEN ? d : opposite(d));
         // reduced-precision storage keeps the moments of MODE_COLL in mout
-        // the fused step's planes read at the cell's own k take the streaming hint
+        // the fused step's LIJK planes read at the cell's own k take the streaming
+        // hint (in IJKL every line holds slots of several cells: no hint)
         if constexpr (PR::kShift && MODE == MODE_COLL) v[d] = mout[a];
         else if constexpr (MODE == MODE_FUSED && PR::kShift) {
-            if (twist_dz(d)) v[d] = load_dev<PR>(Q, a, d);
+            if (L != LayoutLIJK || twist_dz(d)) v[d] = load_dev<PR>(Q, a, d);
             else v[d] = load_dev<PR, true>(Q, a, d);
         } else if constexpr (MODE == MODE_FUSED) {
-            if (twist_dz(d)) v[d] = load_pop<PR>(Q, a, d, rho0);
+            if (L != LayoutLIJK || twist_dz(d)) v[d] = load_pop<PR>(Q, a, d, rho0);
             else v[d] = load_pop<PR, true>(Q, a, d, rho0);
         }
         else v[d] = load_pop<PR>(Q, a, d, rho0);
@@ -323,7 +326,7 @@ __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, ty
         const uint32_t h = n2 + (uint32_t)((twist_dx(e) ? di2 : 0) + (twist_dy(e) ? dj2 : 0) + (twist_dz(e) ? dk2 : 0));
         const uint64_t a = qaddr<L>(g, h, EVEN ? e : opposite(e));
         if constexpr (MODE == MODE_FUSED) {
-            if (twist_dz(e)) {
+            if (L != LayoutLIJK || twist_dz(e)) {
                 if constexpr (PR::kShift) store_dev<PR>(Q, a, d, p[d]);
                 else store_pop<PR>(Q, a, d, p[d], rho0);
             } else {
