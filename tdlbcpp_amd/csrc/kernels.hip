@@ -129,17 +129,13 @@ struct RawBits {
 // two accesses of one address and drop the hint).
 template <bool NT, typename S>
 __device__ __forceinline__ S load_hint(const S* __restrict__ p) {
-    if constexpr (!NT) return *p;
-    else if constexpr (sizeof(S) == 2)
-        return __builtin_bit_cast(S, __builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(p)));
-    else return __builtin_nontemporal_load(p);
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
 }
 template <bool NT, typename S>
 __device__ __forceinline__ void store_hint(S* __restrict__ p, S v) {
-    if constexpr (!NT) *p = v;
-    else if constexpr (sizeof(S) == 2)
-        __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, v), reinterpret_cast<uint16_t*>(p));
-    else __builtin_nontemporal_store(v, p);
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
 }
 
 template <typename PR, bool NT = false>
