@@ -120,10 +120,14 @@ struct RawBits {
 // lines that no other wavefront touches: those accesses carry the
 // non-temporal hint (nt).  The planes it reads at k+1 share their edge line
 // with the next wavefront, which finds it in the cache only without the hint.
-// Measured at 512^3 (FP32): hint on the unshifted planes 4.4 % faster than no
-// hint; on loads only 1.8 % slower; on every access 6 % slower; in IJKL (a
-// line holds slots of several cells) 35 % slower, so IJKL takes no hint.  The
-// Simple pull showed no difference either way and takes none.  The hint is a
+// This holds only when every k-row of a slot plane is a whole number of lines:
+// otherwise a row's first and last lines are shared with the neighbouring rows'
+// blocks and the hint costs a refetch (line_rows).  Measured (Esotwist, LIJK):
+// 512^3 FP32 4.4 % faster with the hint on the unshifted planes (on loads only
+// 1.8 % slower, on every access 6 % slower), 320^3 6 % faster in FP32 and FP16;
+// rows off the line grid 9-27 % slower (200^3, and 160^3 / 96^3 in 16 bits).
+// IJKL (a line holds slots of several cells) is 35 % slower with it; the Simple
+// pull shows no difference; batches, whose tiles mix tanks, take no hint.  The hint is a
 // template argument: each instantiation holds one kind of access, and call sites
 // pick the instantiation per plane (a runtime flag lets the compiler merge the
 // two accesses of one address and drop the hint).
@@ -168,6 +172,13 @@ __device__ __forceinline__ void store_pop(typename PR::S* __restrict__ Q, uint64
 // populations c0 / k (es_collide_dev): f - c0 / k = dec(s) / k and the stored
 // value of a deviation p' is k p', both exact (k a power of two), so the
 // 16-bit rounding is the only one between collisions.
+// k-rows of the population planes start and end on 128-byte lines
+template <typename S>
+inline bool line_rows(const Grid& g) {
+    return ((uint64_t)g.qz * sizeof(S)) % 128 == 0 && ((uint64_t)g.qsize * sizeof(S)) % 128 == 0 &&
+           ((uint64_t)g.qyz * sizeof(S)) % 128 == 0;
+}
+
 template <typename PR, bool NT = false>
 __device__ __forceinline__ float load_dev(const typename PR::S* __restrict__ Q, uint64_t a, int d) {
     return PR::dec_dev(load_hint<NT>(Q + a), d);
@@ -247,7 +258,7 @@ __global__ void k_copy_bytes(uint8_t* __restrict__ d, const uint8_t* __restrict_
 // where d was read.
 // mout receives the moments of MODE_MOM: Q itself (in place) when populations
 // are stored as T, a separate T array in the reference layout otherwise.
-template <typename PR, int L, int MODE, int EVEN>
+template <typename PR, int L, int MODE, int EVEN, bool HINT = false>
 __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, typename PR::T* mout,
                                               const typename PR::T* __restrict__ F, typename PR::N* __restrict__ Nu,
                                               const uint64_t* __restrict__ fmask, const Grid& g,
@@ -271,14 +282,13 @@ __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, ty
     for (int d = 0; d < kQ; d++) {
         const uint64_t a = qaddr<L>(g, home[d], EVEN ? d : opposite(d));
         // reduced-precision storage keeps the moments of MODE_COLL in mout
-        // the fused step's LIJK planes read at the cell's own k take the streaming
-        // hint (in IJKL every line holds slots of several cells: no hint)
+        // with HINT, the planes read at the cell's own k take the streaming hint
         if constexpr (PR::kShift && MODE == MODE_COLL) v[d] = mout[a];
         else if constexpr (MODE == MODE_FUSED && PR::kShift) {
-            if (L != LayoutLIJK || twist_dz(d)) v[d] = load_dev<PR>(Q, a, d);
+            if (!HINT || twist_dz(d)) v[d] = load_dev<PR>(Q, a, d);
             else v[d] = load_dev<PR, true>(Q, a, d);
         } else if constexpr (MODE == MODE_FUSED) {
-            if (L != LayoutLIJK || twist_dz(d)) v[d] = load_pop<PR>(Q, a, d, rho0);
+            if (!HINT || twist_dz(d)) v[d] = load_pop<PR>(Q, a, d, rho0);
             else v[d] = load_pop<PR, true>(Q, a, d, rho0);
         }
         else v[d] = load_pop<PR>(Q, a, d, rho0);
@@ -322,7 +332,7 @@ __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, ty
         const uint32_t h = n2 + (uint32_t)((twist_dx(e) ? di2 : 0) + (twist_dy(e) ? dj2 : 0) + (twist_dz(e) ? dk2 : 0));
         const uint64_t a = qaddr<L>(g, h, EVEN ? e : opposite(e));
         if constexpr (MODE == MODE_FUSED) {
-            if (L != LayoutLIJK || twist_dz(e)) {
+            if (!HINT || twist_dz(e)) {
                 if constexpr (PR::kShift) store_dev<PR>(Q, a, d, p[d]);
                 else store_pop<PR>(Q, a, d, p[d], rho0);
             } else {
@@ -334,7 +344,7 @@ __device__ __forceinline__ void esotwist_cell(typename PR::S* __restrict__ Q, ty
     }
 }
 
-template <typename PR, int L, int MODE, int EVEN>
+template <typename PR, int L, int MODE, int EVEN, bool HINT>
 __global__ __launch_bounds__(kRowBlock) void k_esotwist(typename PR::S* __restrict__ Q, typename PR::T* mout,
                                                      const typename PR::T* __restrict__ F,
                                                      typename PR::N* __restrict__ Nu,
@@ -343,7 +353,7 @@ __global__ __launch_bounds__(kRowBlock) void k_esotwist(typename PR::S* __restri
     const RowBlock rb = row_block(g, layout_block<L>(), nkb, blockDim.x);
     const int k = rb.k0 + (int)threadIdx.x;
     if (k > g.zg1) return;
-    esotwist_cell<PR, L, MODE, EVEN>(Q, mout, F, Nu, fmask, g, P, rho0, rb.i, rb.j, k);
+    esotwist_cell<PR, L, MODE, EVEN, HINT>(Q, mout, F, Nu, fmask, g, P, rho0, rb.i, rb.j, k);
 }
 
 // ------------------------------------------------------------------ Simple
@@ -1241,11 +1251,19 @@ hipError_t esotwist_t(int mode, int even, void* Q, void* mout, const void* F, vo
     const T* f = static_cast<const T*>(F);
     auto* nu = static_cast<typename PR::N*>(Nu);
     const auto P = to_params<T, typename PR::D>(c);
-#define TDLB_ESO(M, E) \
-    hipLaunchKernelGGL((k_esotwist<PR, L, M, E>), dim3(nb), dim3(bk), 0, s, q, mo, f, nu, fm, g, P, nkb, rho0)
-    if (mode == MODE_COLL) { if (even) TDLB_ESO(MODE_COLL, 1); else TDLB_ESO(MODE_COLL, 0); }
-    else if (mode == MODE_MOM) { if (even) TDLB_ESO(MODE_MOM, 1); else TDLB_ESO(MODE_MOM, 0); }
-    else if (mode == MODE_FUSED) { if (even) TDLB_ESO(MODE_FUSED, 1); else TDLB_ESO(MODE_FUSED, 0); }
+#define TDLB_ESO(M, E, H) \
+    hipLaunchKernelGGL((k_esotwist<PR, L, M, E, H>), dim3(nb), dim3(bk), 0, s, q, mo, f, nu, fm, g, P, nkb, rho0)
+    if (mode == MODE_COLL) { if (even) TDLB_ESO(MODE_COLL, 1, false); else TDLB_ESO(MODE_COLL, 0, false); }
+    else if (mode == MODE_MOM) { if (even) TDLB_ESO(MODE_MOM, 1, false); else TDLB_ESO(MODE_MOM, 0, false); }
+    else if (mode == MODE_FUSED) {
+        if constexpr (L == LayoutLIJK) {
+            if (line_rows<S>(g)) {
+                if (even) TDLB_ESO(MODE_FUSED, 1, true); else TDLB_ESO(MODE_FUSED, 0, true);
+                return hipGetLastError();
+            }
+        }
+        if (even) TDLB_ESO(MODE_FUSED, 1, false); else TDLB_ESO(MODE_FUSED, 0, false);
+    }
     else return hipErrorInvalidValue;
 #undef TDLB_ESO
     return hipGetLastError();
