@@ -101,3 +101,23 @@ def test_batch_rejects_mismatched_members():
     with pytest.raises(td.TdlbError):
         td.Batch(td.PREC_F64, td.LAYOUT_LIJK, td.STREAM_ESOTWIST)
     b.close()
+
+
+@pytest.mark.parametrize("prec", [td.PREC_F32, td.PREC_F16, td.PREC_E3M12], ids=["fp32", "fp16", "e3m12"])
+def test_cache_hinted_solo_run_equals_unhinted_batch_member(prec):
+    """At 64^3 every k-row of the LIJK planes is a whole number of 128-byte lines,
+    so tdlb_cu_run launches the fused kernel built with the non-temporal cache
+    hint; batch members never take it.  The hint only changes cache policy: the
+    two runs of one tank must agree bit for bit."""
+    T, steps = 64, 12
+    case(f"cache-hinted solo vs batch member T={T} prec={prec}")
+    solo = make_tank(T, prec, td.STREAM_ESOTWIST, td.LAYOUT_LIJK)
+    mem = make_tank(T, prec, td.STREAM_ESOTWIST, td.LAYOUT_LIJK)
+    b = td.Batch(prec, td.LAYOUT_LIJK, td.STREAM_ESOTWIST)
+    b.add(mem, steps)
+    assert b.run(steps) == steps
+    solo.run(steps)
+    assert (mem.step, mem.angle) == (solo.step, solo.angle)
+    for x, y in zip(fields(mem), fields(solo)):
+        np.testing.assert_array_equal(x, y)
+    b.close()
