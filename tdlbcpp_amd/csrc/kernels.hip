@@ -127,10 +127,10 @@ struct RawBits {
 // 1.8 % slower, on every access 6 % slower), 320^3 6 % faster in FP32 and FP16;
 // rows off the line grid 9-27 % slower (200^3, and 160^3 / 96^3 in 16 bits).
 // IJKL (a line holds slots of several cells) is 35 % slower with it; the Simple
-// pull shows no difference; batches, whose tiles mix tanks, take no hint.  The hint is a
-// template argument: each instantiation holds one kind of access, and call sites
-// pick the instantiation per plane (a runtime flag lets the compiler merge the
-// two accesses of one address and drop the hint).
+// pull shows no difference; batches, whose tiles mix tanks, take no hint.  The
+// hint is a template argument: each instantiation holds one kind of access, and
+// call sites pick the instantiation per plane (a runtime flag lets the compiler
+// merge the two accesses of one address and drop the hint).
 template <bool NT, typename S>
 __device__ __forceinline__ S load_hint(const S* __restrict__ p) {
     if constexpr (NT) return __builtin_nontemporal_load(p);
@@ -168,10 +168,6 @@ __device__ __forceinline__ void store_pop(typename PR::S* __restrict__ Q, uint64
     if constexpr (PR::kShift) store_hint<NT>(Q + a, PR::enc(fmaf(v, shift_scale(d), -shift_offset(rho0))));
     else store_hint<NT>(Q + a, v);
 }
-// The fused collision of the 16-bit formats works on deviations from the rest
-// populations c0 / k (es_collide_dev): f - c0 / k = dec(s) / k and the stored
-// value of a deviation p' is k p', both exact (k a power of two), so the
-// 16-bit rounding is the only one between collisions.
 // k-rows of the population planes start and end on 128-byte lines
 template <typename S>
 inline bool line_rows(const Grid& g) {
@@ -179,6 +175,10 @@ inline bool line_rows(const Grid& g) {
            ((uint64_t)g.qyz * sizeof(S)) % 128 == 0;
 }
 
+// The fused collision of the 16-bit formats works on deviations from the rest
+// populations c0 / k (es_collide_dev): f - c0 / k = dec(s) / k and the stored
+// value of a deviation p' is k p', both exact (k a power of two), so the
+// 16-bit rounding is the only one between collisions.
 template <typename PR, bool NT = false>
 __device__ __forceinline__ float load_dev(const typename PR::S* __restrict__ Q, uint64_t a, int d) {
     return PR::dec_dev(load_hint<NT>(Q + a), d);
