@@ -10,7 +10,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# TDLB_ORACLE_LIB names another build of the same source (the mutation test of
+# tests/test_oracle_mutants.py loads oracles with planted mistakes this way)
+LIB_PATH = os.environ.get("TDLB_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 F32, F64, TAGS = 0, 1, 2
 IJKL, LIJK = 0, 1
