@@ -113,6 +113,8 @@ def lib():
             "tdlb_cu_rotating_points": ([vp, vp, i64, P(i64)], i32),
             "tdlb_cu_solid_mask": ([vp, vp, ctypes.c_size_t], i32),
             "tdlb_cu_forcing_info": ([vp, P(i64), P(i64), P(i64)], i32),
+            "tdlb_cu_one_launch": ([vp, P(i32)], i32),
+            "tdlb_cu_one_launch_phases": ([vp, P(d), i32], i32),
             "tdlb_cu_profile": ([vp, i32], i32),
             "tdlb_cu_profile_read": ([vp, P(d), P(i64), P(d), P(i64)], i32),
             "tdlb_cu_reductions": ([vp, P(d)], i32),
@@ -129,6 +131,8 @@ def lib():
             "tdlb_batch_run": ([vp, i64, P(i64)], i32),
             "tdlb_batch_member": ([vp, vp, P(u64), P(d), P(i64)], i32),
             "tdlb_batch_info": ([vp, P(i64), P(i64), P(ctypes.c_int32), i64], i32),
+            "tdlb_batch_tune": ([vp, vp], i32),
+            "tdlb_batch_shape": ([vp, P(ctypes.c_int32), P(ctypes.c_int32)], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -362,6 +366,20 @@ class ComputeUnit:
                                          ctypes.byref(d)))
         return a.value, b.value, c.value, d.value
 
+    @property
+    def one_launch(self):
+        """True when run() advances this bound tank with the one-launch step."""
+        v = ctypes.c_int32()
+        check(lib().tdlb_cu_one_launch(self._h, ctypes.byref(v)))
+        return bool(v.value)
+
+    def one_launch_phases(self):
+        """Per XCD (rows written, forcing gathers done, forcing levels done), in
+        microseconds from the first work item of the last one-launch step."""
+        out = (ctypes.c_double * 24)()
+        check(lib().tdlb_cu_one_launch_phases(self._h, out, 24))
+        return [tuple(out[3 * x:3 * x + 3]) for x in range(8)]
+
     def forcing_info(self):
         a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         check(lib().tdlb_cu_forcing_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
@@ -445,11 +463,18 @@ class Batch:
             cu.step, cu.angle, _ = self.member(cu)
         return n.value
 
+    def tune(self, cu):
+        """Autotune cu's grid shape on cu's own arrays (before cu.initialise)."""
+        check(lib().tdlb_batch_tune(self._b, cu._h))
+
     def info(self):
         n, l = ctypes.c_int64(), ctypes.c_int64()
         tiles = (ctypes.c_int32 * max(1, len(self.members)))()
         check(lib().tdlb_batch_info(self._b, ctypes.byref(n), ctypes.byref(l), tiles, len(self.members)))
-        return {"members": n.value, "launches_per_step": l.value, "tiles": list(tiles)[:len(self.members)]}
+        bs, gs = ctypes.c_int32(), ctypes.c_int32()
+        check(lib().tdlb_batch_shape(self._b, ctypes.byref(bs), ctypes.byref(gs)))
+        return {"members": n.value, "launches_per_step": l.value, "tiles": list(tiles)[:len(self.members)],
+                "block_size": bs.value, "graph_steps": gs.value}
 
     def close(self):
         if getattr(self, "_b", None) and self._b.value:

@@ -8,13 +8,17 @@
 // tdlb_cu_run on the same handle (same device functions, same order), so a
 // tank's result does not depend on the batch it ran in.
 //
-// Per-member tile shapes of the fused kernel come from an autotuner that times
-// each candidate on scratch lattices of the member's size (never on the
-// member's own state) and caches the choice per grid shape.
+// Tile shapes and the block size of the fused kernel come from an autotuner
+// that times every candidate (3 tiles x 256 / 512 threads) per grid shape on
+// scratch lattices of that shape, or on a handle's own arrays before it is
+// initialised (tdlb_batch_tune), never on a member's state; the timings are
+// cached per shape.  A run takes the block size with the lower total over its
+// members and each member's fastest tile at that size.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
@@ -34,6 +38,7 @@ struct Member {
     double angle = 0.0;  // RunningParams.angle
     int64_t remaining = 0;
     int32_t tile = TILE_ROW64;
+    float ms[2][kTiles] = {};  // autotuned fused-kernel time per block size (256, 512) and tile
     // filled per run
     int64_t call = 0;     // steps left for this member in the current tdlb_batch_run
     int64_t n_steps = 0;
@@ -62,8 +67,13 @@ struct DevBuf {
 };
 
 using TileKey = std::tuple<int64_t, int64_t, int64_t, int, int, int>;
+struct TileTimes {
+    float ms[2][kTiles];
+};
 std::mutex g_tile_mu;
-std::map<TileKey, int32_t> g_tiles;  // autotuned tile per grid shape, precision, layout, streaming
+std::map<TileKey, TileTimes> g_tiles;  // fused-kernel timings per grid shape, precision, layout, streaming
+constexpr int kBlockSizes[2] = {256, 512};
+constexpr int kGraphSteps = 8;  // steps in one replay of the long graph
 
 }  // namespace
 
@@ -77,18 +87,18 @@ struct tdlb_batch {
 
     DevBuf<MemberDev> d_mem;
     DevBuf<int64_t> d_tab;
-    DevBuf<LevelSeg> d_segs;
+    DevBuf<PlaneTask> d_tasks;
     DevBuf<BladeStep> d_steps;
 
     // plan of the last run and its graph
     BatchPlan plan{};
-    std::vector<int64_t> lv_static, lv_blade, lv_threads;
     std::vector<std::pair<tdlb_cu*, int32_t>> key;  // members (with budget) and their tiles
     std::vector<int64_t> launch_sig;                 // launch sizes and arguments of the captured step
     bool buffers_moved = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph[2] = {nullptr, nullptr};     // one step, kGraphSteps steps
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
     int64_t launches_per_step = 0;
+    int32_t bs = 256;                                // fused-kernel block size of the plan
     std::vector<int> active_idx;  // members in the plan (indices into members)
 };
 
@@ -96,11 +106,18 @@ namespace {
 
 double to_prec_f(const tdlb_cu* cu, double v) { return cu->tsize == 4 ? (double)(float)v : v; }
 
+int set_device_of(const tdlb_cu* cu) {
+    HIPCHK(hipSetDevice(cu->device));
+    return TDLB_OK;
+}
+
 void drop_graph(tdlb_batch* b) {
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
+    for (int a = 0; a < 2; a++) {
+        if (b->exec[a]) (void)hipGraphExecDestroy(b->exec[a]);
+        if (b->graph[a]) (void)hipGraphDestroy(b->graph[a]);
+        b->exec[a] = nullptr;
+        b->graph[a] = nullptr;
+    }
 }
 
 MemberDev describe(const tdlb_batch* b, const Member& m) {
@@ -124,25 +141,20 @@ MemberDev describe(const tdlb_batch* b, const Member& m) {
     md.fmask[0] = cu->fmask[0];
     md.fmask[1] = cu->fmask[1];
     md.static_base = cu->static_base;
-    md.pts = cu->pts.d;
-    md.n_static = cu->n_static;
-    md.n_rot = cu->n_rot;
-    md.scells = cu->static_set.cells;
-    md.sidx = cu->static_set.sidx;
-    md.n_scells = cu->static_set.n_cells;
-    md.scache = cu->static_set.cache;
-    md.rcache = cu->rot_cache;
-    md.blade = cu->blade;
+    md.fg = cu->fgeom();
+    md.fg.local_teams = 0;  // the batch orders workgroups by member, not by XCD
     md.even = cu->even ? 1 : 0;
     md.cur = cu->cur;
     md.fm = cu->fm;
     return md;
 }
 
-// Tile of the fused kernel for one member: cached per shape, else timed on
-// scratch lattices of that shape (zero populations; the timing does not depend
-// on the values) when autotuning is on, else the row tile.
-int tune_tile(tdlb_batch* b, const tdlb_cu* cu, int32_t* out) {
+// Timings of the fused kernel for cu's grid shape: cached per shape, else (with
+// autotuning on) timed for every tile and block size on scratch lattices of
+// that shape, or on the handle's own arrays when own (zeroed first: the handle
+// is about to be initialised).  Zero timings (autotuning off) select the row
+// tile at 256 threads.
+int tune_shape(tdlb_batch* b, tdlb_cu* cu, bool own, TileTimes* out, hipStream_t s) {
     const TileKey key{cu->g.xg1, cu->g.yg1, cu->g.zg1, b->prec, b->layout, b->streaming};
     {
         std::lock_guard<std::mutex> lk(g_tile_mu);
@@ -152,81 +164,87 @@ int tune_tile(tdlb_batch* b, const tdlb_cu* cu, int32_t* out) {
             return TDLB_OK;
         }
     }
-    if (!b->autotune) {
-        *out = TILE_ROW64;
-        return TDLB_OK;
-    }
+    *out = TileTimes{};
+    if (!b->autotune) return TDLB_OK;
     const size_t nq = cu->simple() ? 2 : 1;
     void* q[2] = {nullptr, nullptr};
     void *F = nullptr, *Nu = nullptr, *fm = nullptr, *dmem = nullptr, *dtab = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = TDLB_OK;
-    float best = 1e30f;
-    int32_t best_tile = TILE_ROW64;
     auto cleanup = [&]() {
-        for (void* p : {q[0], q[1], F, Nu, fm, dmem, dtab})
+        if (!own)
+            for (void* p : {q[0], q[1], F, Nu, fm})
+                if (p) (void)hipFree(p);
+        for (void* p : {dmem, dtab})
             if (p) (void)hipFree(p);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
     };
     bool ok = true;
-    for (size_t a = 0; a < nq && ok; a++) ok = hipMalloc(&q[a], cu->qs_bytes()) == hipSuccess;
-    ok = ok && hipMalloc(&F, cu->f_bytes()) == hipSuccess && hipMalloc(&Nu, cu->nu_bytes()) == hipSuccess &&
-         hipMalloc(&fm, cu->mask_bytes()) == hipSuccess && hipMalloc(&dmem, sizeof(MemberDev)) == hipSuccess &&
+    if (own) {
+        q[0] = cu->Q[0];
+        q[1] = cu->Q[nq - 1];
+        F = cu->F;
+        Nu = cu->Nu;
+        fm = cu->fmask[0];
+    } else {
+        for (size_t a = 0; a < nq && ok; a++) ok = hipMalloc(&q[a], cu->qs_bytes()) == hipSuccess;
+        ok = ok && hipMalloc(&F, cu->f_bytes()) == hipSuccess && hipMalloc(&Nu, cu->nu_bytes()) == hipSuccess &&
+             hipMalloc(&fm, cu->mask_bytes()) == hipSuccess;
+    }
+    ok = ok && hipMalloc(&dmem, sizeof(MemberDev)) == hipSuccess &&
          hipMalloc(&dtab, 2 * sizeof(int64_t)) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
          hipEventCreate(&e1) == hipSuccess;
     if (!ok) {
         cleanup();
-        return fail(TDLB_ERR_NOMEM, "autotune scratch lattices");
+        return fail(TDLB_ERR_NOMEM, "autotune lattices");
     }
-    for (size_t a = 0; a < nq; a++) ok = ok && launch_zero(q[a], cu->qs_bytes(), b->stream) == hipSuccess;
-    ok = ok && launch_zero(F, cu->f_bytes(), b->stream) == hipSuccess &&
-         launch_zero(Nu, cu->nu_bytes(), b->stream) == hipSuccess &&
-         launch_zero(fm, cu->mask_bytes(), b->stream) == hipSuccess;
-    for (int32_t tile : {(int32_t)TILE_ROW64, (int32_t)TILE_PLANE}) {
-        if (!ok) break;
-        MemberDev md{};
-        md.g = cu->g;
-        md.P = b->prec == TDLB_PREC_F32 ? make_collide_params<float, double>(1e-3f, 0.1f, 0.8f, 1)
-                                        : make_collide_params<float, float>(1e-3f, 0.1f, 0.8f, 1);
-        md.rho0 = 1.0f;
-        md.tile = tile;
-        md.Q[0] = q[0];
-        md.Q[1] = q[nq - 1];
-        md.F = static_cast<float*>(F);
-        md.Nu = Nu;
-        md.fmask[0] = md.fmask[1] = static_cast<uint64_t*>(fm);
-        md.active = 1;
-        md.n_steps = 1;
-        const int64_t tab[2] = {0, batch_coll_blocks(cu->g, tile)};
-        BatchPlan p{};
-        p.mem = static_cast<const MemberDev*>(dmem);
-        p.n_members = 1;
-        p.off_coll = static_cast<const int64_t*>(dtab);
-        p.tot_coll = tab[1];
-        ok = hipMemcpyAsync(dmem, &md, sizeof(md), hipMemcpyHostToDevice, b->stream) == hipSuccess &&
-             hipMemcpyAsync(dtab, tab, sizeof(tab), hipMemcpyHostToDevice, b->stream) == hipSuccess;
-        for (int w = 0; w < 2 && ok; w++)
-            ok = launch_batch_fused(b->prec, b->dlayout(), b->streaming, p, b->stream) == hipSuccess;
-        ok = ok && hipEventRecord(e0, b->stream) == hipSuccess;
-        constexpr int reps = 5;
-        for (int w = 0; w < reps && ok; w++)
-            ok = launch_batch_fused(b->prec, b->dlayout(), b->streaming, p, b->stream) == hipSuccess;
-        ok = ok && hipEventRecord(e1, b->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
-        float ms = 0.f;
-        ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
-        if (ok && ms < best) {
-            best = ms;
-            best_tile = tile;
+    for (size_t a = 0; a < nq; a++) ok = ok && launch_zero(q[a], cu->qs_bytes(), s) == hipSuccess;
+    ok = ok && launch_zero(F, cu->f_bytes(), s) == hipSuccess && launch_zero(Nu, cu->nu_bytes(), s) == hipSuccess &&
+         launch_zero(fm, cu->mask_bytes(), s) == hipSuccess;
+    TileTimes tt{};
+    for (int bi = 0; bi < 2 && ok; bi++)
+        for (int32_t tile = 0; tile < kTiles && ok; tile++) {
+            MemberDev md{};
+            md.g = cu->g;
+            md.P = b->prec == TDLB_PREC_F32 ? make_collide_params<float, double>(1e-3f, 0.1f, 0.8f, 1)
+                                            : make_collide_params<float, float>(1e-3f, 0.1f, 0.8f, 1);
+            md.rho0 = 1.0f;
+            md.tile = tile;
+            md.Q[0] = q[0];
+            md.Q[1] = q[nq - 1];
+            md.F = static_cast<float*>(F);
+            md.Nu = Nu;
+            md.fmask[0] = md.fmask[1] = static_cast<uint64_t*>(fm);
+            md.active = 1;
+            md.n_steps = 1;
+            const int64_t tab[2] = {0, batch_coll_blocks(cu->g, tile, kBlockSizes[bi])};
+            BatchPlan p{};
+            p.mem = static_cast<const MemberDev*>(dmem);
+            p.n_members = 1;
+            p.off_coll = static_cast<const int64_t*>(dtab);
+            p.tot_coll = tab[1];
+            p.bs = kBlockSizes[bi];
+            ok = hipMemcpyAsync(dmem, &md, sizeof(md), hipMemcpyHostToDevice, s) == hipSuccess &&
+                 hipMemcpyAsync(dtab, tab, sizeof(tab), hipMemcpyHostToDevice, s) == hipSuccess;
+            for (int w = 0; w < 2 && ok; w++)
+                ok = launch_batch_fused(b->prec, b->dlayout(), b->streaming, p, s) == hipSuccess;
+            ok = ok && hipEventRecord(e0, s) == hipSuccess;
+            constexpr int reps = 5;
+            for (int w = 0; w < reps && ok; w++)
+                ok = launch_batch_fused(b->prec, b->dlayout(), b->streaming, p, s) == hipSuccess;
+            ok = ok && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+            float ms = 0.f;
+            ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+            tt.ms[bi][tile] = ms / reps;
         }
-    }
     if (!ok) rc = fail(TDLB_ERR_HIP, "autotune launch failed");
-    (void)hipStreamSynchronize(b->stream);
+    (void)hipStreamSynchronize(s);
     cleanup();
     if (rc != TDLB_OK) return rc;
     std::lock_guard<std::mutex> lk(g_tile_mu);
-    g_tiles[key] = best_tile;
-    *out = best_tile;
+    g_tiles[key] = tt;
+    *out = tt;
     return TDLB_OK;
 }
 
@@ -250,25 +268,49 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     for (size_t a = 0; a < b->members.size(); a++)
         if (b->members[a].call > 0) idx.push_back((int)a);
     const int M = (int)idx.size();
+    // block size with the lower total of the members' fastest tiles, then each member's tile at it
+    {
+        double tot[2] = {0, 0};
+        for (int a : idx)
+            for (int bi = 0; bi < 2; bi++) {
+                const float* t = b->members[a].ms[bi];
+                tot[bi] += *std::min_element(t, t + kTiles);
+            }
+        const int bi = tot[1] < tot[0] ? 1 : 0;
+        b->bs = kBlockSizes[bi];
+        for (int a : idx) {
+            const float* t = b->members[a].ms[bi];
+            b->members[a].tile = (int32_t)(std::min_element(t, t + kTiles) - t);
+        }
+    }
     std::vector<std::pair<tdlb_cu*, int32_t>> key;
     for (int a : idx) key.push_back({b->members[a].cu, b->members[a].tile});
 
     std::vector<MemberDev> mem(M);
-    std::vector<int64_t> tab(4 * (size_t)(M + 1), 0);
+    std::vector<int64_t> tab(3 * (size_t)(M + 1), 0);
     int64_t* off_coll = tab.data();
     int64_t* off_rot = off_coll + (M + 1);
     int64_t* off_words = off_rot + (M + 1);
-    int64_t* off_scells = off_words + (M + 1);
     std::vector<BladeStep> steps;
     std::vector<size_t> step_base(M);
+    // forcing workgroups: the teams of every member first (a team's workgroups
+    // must be resident together), then every solo plane, longest first
+    std::vector<PlaneTask> team_tasks;
+    std::vector<std::pair<float, PlaneTask>> tk;
     for (int r = 0; r < M; r++) {
         Member& m = b->members[idx[r]];
         tdlb_cu* cu = m.cu;
         m.n_steps = std::min<int64_t>(m.call, K);
-        off_coll[r + 1] = off_coll[r] + batch_coll_blocks(cu->g, m.tile);
+        off_coll[r + 1] = off_coll[r] + batch_coll_blocks(cu->g, m.tile, b->bs);
         off_rot[r + 1] = off_rot[r] + cu->n_rot;
         off_words[r + 1] = off_words[r] + (int64_t)cu->mask_words;
-        off_scells[r + 1] = off_scells[r] + cu->static_set.n_cells;
+        const auto& bl = cu->static_set.block_list;
+        const auto& pl = cu->static_set.plane_list;
+        for (int32_t b = 0; b < (int32_t)bl.size(); b++) {
+            if (bl[b].plane < 0) continue;  // an empty slot of the single-tank launch
+            if (pl[bl[b].plane].team > 1) team_tasks.push_back(PlaneTask{r, b});
+            else tk.push_back({cu->static_set.plane_cost[bl[b].plane], PlaneTask{r, b}});
+        }
         // blade orientation of each step (the single-handle loop computes the same)
         Rushton R(cu->rp);
         m.angles.resize(m.n_steps);
@@ -290,55 +332,25 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
         mem[r].active = m.n_steps > 0 ? 1 : 0;
         mem[r].done = 0;
     }
-    // forcing levels: level l of every member that has it
-    std::vector<LevelSeg> segs;
-    std::vector<int64_t> cum;
-    b->lv_threads.clear();
-    // lv[l] .. lv[l + 1] are the segments of level l; the blade levels follow the static ones in segs
-    auto add_levels = [&](bool stat, std::vector<int64_t>& lv) {
-        lv.assign(1, (int64_t)segs.size());
-        size_t nl = 0;
-        for (int r = 0; r < M; r++) {
-            const tdlb_cu* cu = b->members[idx[r]].cu;
-            const auto& L = stat ? cu->static_levels : cu->rot_levels;
-            if (L.size() > 1) nl = std::max(nl, L.size() - 1);
-        }
-        for (size_t l = 0; l < nl; l++) {
-            int64_t threads = 0;
-            for (int r = 0; r < M; r++) {
-                const tdlb_cu* cu = b->members[idx[r]].cu;
-                const auto& L = stat ? cu->static_levels : cu->rot_levels;
-                if (l + 1 >= L.size()) continue;
-                const int64_t cnt = L[l + 1] - L[l];
-                if (cnt <= 0) continue;
-                segs.push_back(LevelSeg{r, 0, L[l], cnt});
-                cum.push_back(threads);
-                threads += cnt;
-            }
-            lv.push_back((int64_t)segs.size());
-            b->lv_threads.push_back(threads);
-        }
-    };
-    add_levels(true, b->lv_static);
-    add_levels(false, b->lv_blade);
-    const int64_t tot_coll = off_coll[M], tot_rot = off_rot[M], tot_words = off_words[M], tot_scells = off_scells[M];
-    tab.insert(tab.end(), cum.begin(), cum.end());  // may move tab: the off_* pointers are not used below
-    const size_t ntab = tab.size();
+    std::stable_sort(tk.begin(), tk.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+    std::vector<PlaneTask> tasks(team_tasks);
+    for (const auto& t : tk) tasks.push_back(t.second);
+    const int64_t tot_coll = off_coll[M], tot_rot = off_rot[M], tot_words = off_words[M];
 
     int moved = 0;
     int g;
     if ((g = b->d_mem.grow(std::max(M, 1))) < 0) return fail(TDLB_ERR_NOMEM, "batch descriptors");
     moved |= g;
-    if ((g = b->d_tab.grow(ntab)) < 0) return fail(TDLB_ERR_NOMEM, "batch tables");
+    if ((g = b->d_tab.grow(tab.size())) < 0) return fail(TDLB_ERR_NOMEM, "batch tables");
     moved |= g;
-    if ((g = b->d_segs.grow(std::max<size_t>(segs.size(), 1))) < 0) return fail(TDLB_ERR_NOMEM, "batch levels");
+    if ((g = b->d_tasks.grow(std::max<size_t>(tasks.size(), 1))) < 0) return fail(TDLB_ERR_NOMEM, "batch tasks");
     moved |= g;
     if (b->d_steps.grow(std::max<size_t>(steps.size(), 1)) < 0) return fail(TDLB_ERR_NOMEM, "batch blade table");
     for (int r = 0; r < M; r++) mem[r].steps = b->d_steps.p + step_base[r];
     if (M > 0) HIPCHK(hipMemcpyAsync(b->d_mem.p, mem.data(), M * sizeof(MemberDev), hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipMemcpyAsync(b->d_tab.p, tab.data(), tab.size() * sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
-    if (!segs.empty())
-        HIPCHK(hipMemcpyAsync(b->d_segs.p, segs.data(), segs.size() * sizeof(LevelSeg), hipMemcpyHostToDevice,
+    if (!tasks.empty())
+        HIPCHK(hipMemcpyAsync(b->d_tasks.p, tasks.data(), tasks.size() * sizeof(PlaneTask), hipMemcpyHostToDevice,
                               b->stream));
     if (!steps.empty())
         HIPCHK(hipMemcpyAsync(b->d_steps.p, steps.data(), steps.size() * sizeof(BladeStep), hipMemcpyHostToDevice,
@@ -352,29 +364,20 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     p.off_coll = b->d_tab.p;
     p.off_rot = b->d_tab.p + (M + 1);
     p.off_words = b->d_tab.p + 2 * (M + 1);
-    p.off_scells = b->d_tab.p + 3 * (M + 1);
-    p.seg_cum = b->d_tab.p + 4 * (M + 1);
-    p.segs = b->d_segs.p;
+    p.tasks = b->d_tasks.p;
     p.tot_coll = tot_coll;
     p.tot_rot = tot_rot;
     p.tot_words = tot_words;
-    p.tot_scells = tot_scells;
-    p.n_slevels = (int32_t)b->lv_static.size() - 1;
-    p.n_rlevels = (int32_t)b->lv_blade.size() - 1;
-    p.lv_static = b->lv_static.data();
-    p.lv_blade = b->lv_blade.data();
-    p.lv_threads = b->lv_threads.data();
-    // launches: blades, fused, mask, prepare, static moments, blade moments, levels, advance
-    b->launches_per_step = 1 + 1 + 1 + 1 + 1 + 1 + p.n_slevels + p.n_rlevels + 1;
+    p.n_tasks = (int64_t)tasks.size();
+    p.bs = b->bs;
+    // launches: blades, fused, mask, forcing planes, advance
+    b->launches_per_step = 5;
     // A captured step bakes in every launch size and kernel argument: recapture
     // whenever any of them differs (member identity alone is not enough, a freed
     // handle's address can come back for a tank of another size).
-    std::vector<int64_t> sig = {M, tot_coll, tot_rot, tot_words, tot_scells, p.n_slevels, p.n_rlevels,
+    std::vector<int64_t> sig = {M, tot_coll, tot_rot, tot_words, p.n_tasks, p.bs,
                                 (int64_t)(uintptr_t)b->d_mem.p, (int64_t)(uintptr_t)b->d_tab.p,
-                                (int64_t)(uintptr_t)b->d_segs.p};
-    sig.insert(sig.end(), b->lv_static.begin(), b->lv_static.end());
-    sig.insert(sig.end(), b->lv_blade.begin(), b->lv_blade.end());
-    sig.insert(sig.end(), b->lv_threads.begin(), b->lv_threads.end());
+                                (int64_t)(uintptr_t)b->d_tasks.p};
     *structure_changed = moved || sig != b->launch_sig || key != b->key || b->buffers_moved;
     b->buffers_moved = false;
     b->key = key;
@@ -383,24 +386,27 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     return TDLB_OK;
 }
 
-int record_step(tdlb_batch* b) {
-    HIPCHK(launch_batch_step(b->prec, b->dlayout(), b->streaming, b->plan, b->stream));
+int record_steps(tdlb_batch* b, int steps) {
+    HIPCHK(launch_batch_step(b->prec, b->dlayout(), b->streaming, b->plan, steps, b->stream));
     return TDLB_OK;
 }
 
+// graphs of one step and of kGraphSteps steps (the step state lives on the device)
 int capture(tdlb_batch* b) {
     drop_graph(b);
-    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    const int r = record_step(b);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(b->stream, &g);
-    if (r != TDLB_OK) {
-        if (g) (void)hipGraphDestroy(g);
-        return r;
+    for (int a = 0; a < 2; a++) {
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const int r = record_steps(b, a == 0 ? 1 : kGraphSteps);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (r != TDLB_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        b->graph[a] = g;
+        HIPCHK(hipGraphInstantiate(&b->exec[a], g, nullptr, nullptr, 0));
     }
-    if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-    b->graph = g;
-    HIPCHK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
     return TDLB_OK;
 }
 
@@ -443,7 +449,7 @@ int tdlb_batch_destroy(tdlb_batch_t b) {
     drop_graph(b);
     b->d_mem.release();
     b->d_tab.release();
-    b->d_segs.release();
+    b->d_tasks.release();
     b->d_steps.release();
     (void)hipStreamDestroy(b->stream);
     delete b;
@@ -472,8 +478,28 @@ int tdlb_batch_add(tdlb_batch_t b, tdlb_cu_t cu, uint64_t step, double angle, in
     m.step = step;
     m.angle = angle;
     m.remaining = nsteps;
-    CHK(tune_tile(b, cu, &m.tile));
+    TileTimes tt{};
+    CHK(tune_shape(b, cu, false, &tt, b->stream));
+    std::memcpy(m.ms, tt.ms, sizeof(m.ms));
     b->members.push_back(m);
+    b->buffers_moved = true;  // new member: recapture
+    return TDLB_OK;
+}
+
+int tdlb_batch_tune(tdlb_batch_t b, tdlb_cu_t cu) {
+    if (!b || !cu) return fail(TDLB_ERR_INVALID, "null argument");
+    if (cu->prm.precision != b->prec || cu->prm.layout != b->layout || cu->prm.streaming != b->streaming)
+        return fail(TDLB_ERR_INVALID, "precision, layout and streaming must match the batch");
+    if (cu->g.xg1 < 3 || cu->g.zg1 < 3) return fail(TDLB_ERR_UNSUPPORTED, "members need at least 3 cells in x and z");
+    CHK(set_device_of(cu));
+    TileTimes tt{};
+    return tune_shape(b, cu, true, &tt, cu->stream);  // the arrays it wrote are reset by tdlb_cu_initialise
+}
+
+int tdlb_batch_shape(tdlb_batch_t b, int32_t* block_size, int32_t* graph_steps) {
+    if (!b) return fail(TDLB_ERR_INVALID, "null batch");
+    if (block_size) *block_size = b->bs;
+    if (graph_steps) *graph_steps = b->use_graph ? kGraphSteps : 1;
     return TDLB_OK;
 }
 
@@ -520,10 +546,12 @@ int tdlb_batch_run(tdlb_batch_t b, int64_t nsteps, int64_t* steps_run) {
     CHK(build_plan(b, K, &changed));
     if (b->plan.n_members == 0) return TDLB_OK;
     if (b->use_graph) {
-        if (changed || !b->exec) CHK(capture(b));
-        for (int64_t s = 0; s < K; s++) HIPCHK(hipGraphLaunch(b->exec, b->stream));
+        if (changed || !b->exec[0]) CHK(capture(b));
+        int64_t s = 0;
+        for (; s + kGraphSteps <= K; s += kGraphSteps) HIPCHK(hipGraphLaunch(b->exec[1], b->stream));
+        for (; s < K; s++) HIPCHK(hipGraphLaunch(b->exec[0], b->stream));
     } else {
-        for (int64_t s = 0; s < K; s++) CHK(record_step(b));
+        for (int64_t s = 0; s < K; s++) CHK(record_steps(b, 1));
     }
     std::vector<MemberDev> mem(b->plan.n_members);
     HIPCHK(hipMemcpyAsync(mem.data(), b->d_mem.p, mem.size() * sizeof(MemberDev), hipMemcpyDeviceToHost, b->stream));

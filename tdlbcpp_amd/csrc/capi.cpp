@@ -28,7 +28,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <numeric>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -58,7 +61,7 @@ int set_device(tdlb_cu* cu) {
 
 void release(tdlb_cu* cu) {
     void* ptrs[] = {cu->Q[0], cu->Q[1], cu->M, cu->F, cu->Nu, cu->O, cu->fmask[0], cu->fmask[1],
-                    cu->gmask, cu->static_bits, cu->static_base, cu->red};
+                    cu->gmask, cu->static_bits, cu->static_base, cu->red, cu->rowcell, cu->pctr};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     cu->pts.release();
@@ -325,21 +328,322 @@ int forcing_pass(tdlb_cu* cu, const DevPoints& dp, const std::vector<ForceSegmen
     return TDLB_OK;
 }
 
+// cost (estimated microseconds) above which a plane gets a team of workgroups,
+// one per this much; a wall plane of a 512^3 tank is ~100
+constexpr double kTeamCost = 120.0;
+constexpr int32_t kMaxTeam = 16;
+// TDLB_FORCE_TEAM_COST overrides kTeamCost (tests give small tanks teams)
+double team_cost() {
+    const char* v = std::getenv("TDLB_FORCE_TEAM_COST");
+    const double c = v ? std::atof(v) : 0.0;
+    return c > 0.0 ? c : kTeamCost;
+}
+
+// Plane tasks of a bound tank's static list (ForceGeom, kernels.h).  Levels as
+// compute_levels over the list order; the device order is plane by plane, then
+// level by level (stable), so the points of one plane and level are
+// contiguous.  Every interior plane with static or blade points is a task;
+// tasks are ordered by an estimate of their duration, longest first, so the
+// disk plane starts at once.
+int build_plane_set(tdlb_cu* cu, const std::vector<tdlb_ib_point>& stat, const BladeEnum& be, int64_t blade_j0,
+                    std::vector<int64_t>& order, IndexedSet& set) {
+    set.release();
+    const int64_t n = (int64_t)stat.size();
+    const int64_t gh = cu->prm.ghost;
+    std::vector<int64_t> lorder, loffs;
+    compute_levels(cu, stat.data(), n, lorder, loffs);
+    std::vector<int32_t> lev(n);
+    for (size_t l = 0; l + 1 < loffs.size(); l++)
+        for (int64_t a = loffs[l]; a < loffs[l + 1]; a++) lev[lorder[a]] = (int32_t)l;
+    order.resize(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        return stat[a].j != stat[b].j ? stat[a].j < stat[b].j : lev[a] < lev[b];
+    });
+    const int n_classes = be.n_blades > 0 ? 5 * be.n_bcol : 0;
+    const int64_t blade_pts = (int64_t)be.n_blades * be.n_radial;  // per blade plane
+    std::vector<PlaneDesc> planes;
+    std::vector<double> cost;
+    std::vector<int64_t> lev_off;
+    std::vector<uint32_t> cells;
+    std::vector<int32_t> sidx(9 * (size_t)n);
+    std::vector<uint32_t> pc;
+    uint64_t sc[9];
+    int32_t max_levels = 0;
+    int64_t a0 = 0;
+    for (int64_t j = 1; j <= cu->g.yg1; j++) {
+        int64_t a1 = a0;
+        while (a1 < n && stat[order[a1]].j + gh == j) a1++;
+        const bool blades = n_classes > 0 && j >= blade_j0 && j < blade_j0 + be.n_planes;
+        if (a1 == a0 && !blades) {
+            a0 = a1;
+            continue;
+        }
+        PlaneDesc pd{};
+        pd.j = (int32_t)j;
+        pd.cell_begin = (int64_t)cells.size();
+        pc.clear();
+        for (int64_t a = a0; a < a1; a++) {
+            const tdlb_ib_point& q = stat[order[a]];
+            stencil_cells(cu, q.i + gh, q.j + gh, q.k + gh, sc);
+            for (int t = 0; t < 9; t++) pc.push_back((uint32_t)sc[t]);
+        }
+        std::vector<uint32_t> uq(pc);
+        std::sort(uq.begin(), uq.end());
+        uq.erase(std::unique(uq.begin(), uq.end()), uq.end());
+        for (size_t e = 0; e < pc.size(); e++)
+            sidx[9 * (size_t)a0 + e] =
+                (int32_t)(pd.cell_begin + (std::lower_bound(uq.begin(), uq.end(), pc[e]) - uq.begin()));
+        cells.insert(cells.end(), uq.begin(), uq.end());
+        pd.cell_end = (int64_t)cells.size();
+        const int32_t nl = a1 > a0 ? lev[order[a1 - 1]] + 1 : 0;
+        pd.n_levels = nl;
+        pd.lev = (int64_t)lev_off.size();
+        int64_t a = a0, passes = 0;
+        for (int32_t l = 0; l < nl; l++) {
+            lev_off.push_back(a);
+            const int64_t b = a;
+            while (a < a1 && lev[order[a]] == l) a++;
+            passes += (a - b + kForcePlanePoints - 1) / kForcePlanePoints;
+        }
+        lev_off.push_back(a1);
+        max_levels = std::max(max_levels, nl);
+        int64_t gather = pd.cell_end - pd.cell_begin;
+        if (blades) {
+            for (int c = 0; c < n_classes; c++) {
+                const int64_t cls = blade_class_size(be, c) / std::max(be.n_planes, 1);
+                passes += (cls + kForcePlanePoints - 1) / kForcePlanePoints;
+            }
+            gather += 9 * blade_pts;
+        }
+        planes.push_back(pd);
+        cost.push_back(3.0 * (double)(passes + n_classes) + (double)gather / 1024.0);
+        a0 = a1;
+    }
+    if (a0 != n) return fail(TDLB_ERR_INVALID, "static IB points outside the interior planes");
+    // A plane whose passes would keep one workgroup far longer than a wall
+    // plane (the impeller disk's) runs on a team; teams come first in the
+    // launch, then the solo planes, longest first.
+    std::vector<size_t> by(planes.size());
+    std::iota(by.begin(), by.end(), 0);
+    std::stable_sort(by.begin(), by.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+    std::vector<PlaneDesc> sorted(planes.size());
+    std::vector<PlaneBlock> blocks;
+    const double tcost = team_cost();
+    set.plane_cost.resize(planes.size());
+    int32_t n_teams = 0;
+    for (size_t t = 0; t < by.size(); t++) {
+        PlaneDesc pd = planes[by[t]];
+        const double c = cost[by[t]];
+        pd.team = (int32_t)std::min<double>(kMaxTeam, std::max(1.0, std::ceil(c / tcost)));
+        pd.bar = pd.team > 1 ? n_teams++ : -1;
+        sorted[t] = pd;
+        set.plane_cost[t] = (float)(c / pd.team);
+    }
+    // a team's workgroups go kXcds apart (launches dispatch workgroup b to XCD
+    // b mod kXcds), so one XCD runs the whole team; the other slots take the
+    // solo planes, longest first
+    {
+        int64_t row[kXcds] = {0};
+        std::vector<PlaneBlock> slot;
+        for (size_t t = 0; t < sorted.size(); t++) {
+            if (sorted[t].team == 1) continue;
+            int x = 0;
+            for (int y = 1; y < kXcds; y++)
+                if (row[y] < row[x]) x = y;
+            for (int32_t r = 0; r < sorted[t].team; r++) {
+                const size_t at = (size_t)x + (size_t)kXcds * (size_t)(row[x] + r);
+                if (slot.size() <= at) slot.resize(at + 1, PlaneBlock{-1, 0});
+                slot[at] = PlaneBlock{(int32_t)t, r};
+            }
+            row[x] += sorted[t].team;
+        }
+        size_t next_solo = 0;
+        auto solo = [&]() -> PlaneBlock {
+            while (next_solo < sorted.size() && sorted[next_solo].team != 1) next_solo++;
+            return next_solo < sorted.size() ? PlaneBlock{(int32_t)next_solo++, 0} : PlaneBlock{-1, 0};
+        };
+        for (auto& b : slot)
+            if (b.plane < 0) b = solo();
+        for (PlaneBlock b = solo(); b.plane >= 0; b = solo()) slot.push_back(b);
+        blocks = slot;
+    }
+    set.n_cells = (int64_t)cells.size();
+    set.n_points = n;
+    set.n_planes = (int32_t)sorted.size();
+    set.n_blocks = (int32_t)blocks.size();
+    set.n_teams = n_teams;
+    set.max_levels = max_levels;
+    set.block_list = blocks;
+    set.plane_list = sorted;
+    auto upload = [&](void** dst, const void* src, size_t bytes) -> int {
+        if (bytes == 0) return TDLB_OK;
+        if (hipMalloc(dst, bytes) != hipSuccess) return fail(TDLB_ERR_NOMEM, "IB plane tables");
+        HIPCHK(hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, cu->stream));
+        return TDLB_OK;
+    };
+    CHK(upload((void**)&set.cells, cells.data(), cells.size() * 4));
+    CHK(upload((void**)&set.sidx, sidx.data(), sidx.size() * 4));
+    CHK(upload((void**)&set.planes, sorted.data(), sorted.size() * sizeof(PlaneDesc)));
+    CHK(upload((void**)&set.lev_off, lev_off.data(), lev_off.size() * sizeof(int64_t)));
+    CHK(upload((void**)&set.blocks, blocks.data(), blocks.size() * sizeof(PlaneBlock)));
+    if (n_teams > 0) {
+        const size_t bb = (size_t)n_teams * kBarWords * sizeof(uint32_t);
+        if (hipMalloc((void**)&set.bar, bb) != hipSuccess) return fail(TDLB_ERR_NOMEM, "IB team barriers");
+        HIPCHK(launch_zero(set.bar, bb, cu->stream));
+    }
+    if (!cells.empty() && hipMalloc(&set.cache, cells.size() * 4 * cu->tsize) != hipSuccess)
+        return fail(TDLB_ERR_NOMEM, "IB moment cache");
+    HIPCHK(hipStreamSynchronize(cu->stream));  // host tables go out of scope
+    return TDLB_OK;
+}
+
+// Whether this process's GPU can run the one-launch step: workgroups of a
+// launch must see every XCC id (each XCD's work queue needs workers); checked
+// once per device.  The step is opt-in (TDLB_PIPE=1): at 512^3 its rows run
+// slower than the separate stream-collide kernel.
+bool pipe_device_ok(tdlb_cu* cu, bool honour_env = true) {
+    static std::mutex mu;
+    static std::unordered_map<int, bool> known;
+    const char* env = std::getenv("TDLB_PIPE");
+    if (honour_env && !(env && std::atoi(env) == 1)) return false;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = known.find(cu->device);
+    if (it != known.end()) return it->second;
+    const bool ok = pipe_xcc_check(cu->stream) == 1;
+    known[cu->device] = ok;
+    return ok;
+}
+
+// Tables of the one-launch step (PipeArgs): the first static stencil cell of
+// every row in the plane-grouped cell list, and the forcing-level workgroups
+// of each XCD's plane range, teams first.
+int setup_pipe(tdlb_cu* cu) {
+    cu->pipe = false;
+    for (void* p : {(void*)cu->rowcell, (void*)cu->pctr})
+        if (p) (void)hipFree(p);
+    cu->rowcell = nullptr;
+    cu->pctr = nullptr;
+    const Grid& g = cu->g;
+    const bool shape = !cu->simple() && cu->dlay() == LayoutLIJK && cu->prm.precision != TDLB_PREC_F64 &&
+                       g.xg1 >= 4 && g.yg1 >= 3 * kXcds && g.zg1 >= 64;
+    if (!shape || !pipe_device_ok(cu)) return TDLB_OK;
+    const IndexedSet& set = cu->static_set;
+    const int rw = g.xg1 + 2;
+    std::vector<int32_t> rowcell((size_t)(g.yg1 + 2) * rw, 0);
+    std::vector<uint32_t> cells((size_t)set.n_cells);
+    if (!cells.empty())
+        HIPCHK(hipMemcpyAsync(cells.data(), set.cells, cells.size() * 4, hipMemcpyDeviceToHost, cu->stream));
+    HIPCHK(hipStreamSynchronize(cu->stream));
+    // plane j's cells lie in [first of j, first of j+1); rows by cell index within
+    std::vector<int64_t> pb((size_t)g.yg1 + 2, -1), pe((size_t)g.yg1 + 2, -1);
+    for (const PlaneDesc& pd : set.plane_list) {
+        pb[pd.j] = pd.cell_begin;
+        pe[pd.j] = pd.cell_end;
+    }
+    for (int j = 0; j <= g.yg1 + 1; j++) {
+        const int64_t b = pb[j] < 0 ? 0 : pb[j], e = pe[j] < 0 ? 0 : pe[j];
+        for (int i = 1; i <= g.xg1 + 1; i++) {
+            const uint32_t first = (uint32_t)i * g.yz + (uint32_t)j * (uint32_t)g.zg;
+            const int64_t at = std::lower_bound(cells.begin() + b, cells.begin() + e, first) - cells.begin();
+            rowcell[(size_t)j * rw + i] = (int32_t)(i == g.xg1 + 1 ? e : at);
+        }
+    }
+    const size_t words = pipe_counter_words(g);
+    if (hipMalloc((void**)&cu->rowcell, rowcell.size() * 4) != hipSuccess ||
+        hipMalloc((void**)&cu->pctr, words * 4) != hipSuccess)
+        return fail(TDLB_ERR_NOMEM, "one-launch step tables");
+    HIPCHK(hipMemcpyAsync(cu->rowcell, rowcell.data(), rowcell.size() * 4, hipMemcpyHostToDevice, cu->stream));
+    HIPCHK(launch_zero(cu->pctr, words * 4, cu->stream));
+    HIPCHK(hipStreamSynchronize(cu->stream));
+    cu->pipe = true;
+    return TDLB_OK;
+}
+
+// One step of a bound Esotwist tank in one launch: blade points of the step
+// generated before, static stencil cells prepared when the previous mask may
+// lack them.  Same arithmetic and order as launch_esotwist + bound_forcing.
+int pipe_step(tdlb_cu* cu) {
+    uint64_t* prev = cu->fmask[cu->fm];
+    uint64_t* next = cu->fmask[1 - cu->fm];
+    if (!cu->static_in_prev)
+        HIPCHK(launch_force_prepare(cu->eprec(), cu->pts.d, 0, cu->n_static, cu->F, prev, next, cu->g, cu->stream));
+    PipeArgs a{};
+    a.Q = cu->Q[0];
+    a.F = cu->F;
+    a.Nu = cu->Nu;
+    a.prev = prev;
+    a.next = next;
+    a.static_base = cu->static_base;
+    a.fg = cu->fgeom();
+    a.rowcell = cu->rowcell;
+    a.ctr = cu->pctr;
+    HIPCHK(launch_zero(cu->pctr, pipe_counter_words(cu->g) * 4, cu->stream));
+    const int hint = line_rows_bytes(cu->g, cu->ssize) ? 1 : 0;
+    HIPCHK(launch_eso_pipe(cu->prec(), cu->even ? 1 : 0, hint, a, cu->g, cu->cargs(), cu->rho0, cu->alfa, cu->beta,
+                           0, row_lanes(cu->g), cu->stream));
+    if (cu->o_dirty) {
+        HIPCHK(launch_zero_interior_o(cu->O, cu->g, cu->stream));
+        cu->o_dirty = false;
+    }
+    cu->even = !cu->even;
+    cu->fm = 1 - cu->fm;
+    cu->static_in_prev = true;
+    return TDLB_OK;
+}
+
+// A team barrier of the forcing planes that gave up waiting leaves its flag
+// set (kBarSpinLimit): report it once and clear it.  Never expected (teams are
+// resident together); results after it are not the reference's.
+int check_team_barriers(tdlb_cu* cu) {
+    if (cu->pctr) {
+        uint32_t f[kXcds];
+        for (int x = 0; x < kXcds; x++)
+            HIPCHK(hipMemcpy(&f[x], cu->pctr + 64 * x + 4, 4, hipMemcpyDeviceToHost));
+        for (int x = 0; x < kXcds; x++)
+            if (f[x]) {
+                HIPCHK(launch_zero(cu->pctr, pipe_counter_words(cu->g) * 4, cu->stream));
+                HIPCHK(hipStreamSynchronize(cu->stream));
+                return fail(TDLB_ERR_HIP, "one-launch step: a wait timed out (results are invalid)");
+            }
+    }
+    const IndexedSet& set = cu->static_set;
+    if (!set.bar || set.n_teams <= 0) return TDLB_OK;
+    std::vector<uint32_t> w((size_t)set.n_teams * kBarWords);
+    HIPCHK(hipMemcpy(w.data(), set.bar, w.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    uint32_t bad = 0;
+    for (int32_t t = 0; t < set.n_teams; t++) bad |= w[(size_t)t * kBarWords + 2];
+    if (!bad) return TDLB_OK;
+    if (bad & 2u) {
+        HIPCHK(launch_zero(set.bar, w.size() * sizeof(uint32_t), cu->stream));
+        HIPCHK(hipStreamSynchronize(cu->stream));
+        return fail(TDLB_ERR_HIP, "IB forcing: a plane team ran on more than one XCD (forcing results are invalid)");
+    }
+    // a workgroup that gave up left the arrival count behind: restart every barrier
+    HIPCHK(launch_zero(set.bar, w.size() * sizeof(uint32_t), cu->stream));
+    HIPCHK(hipStreamSynchronize(cu->stream));
+    return fail(TDLB_ERR_HIP, "IB forcing: a plane team barrier timed out (forcing results are invalid)");
+}
+
+// One forcing pass of the bound tank: the next occupancy mask starts from the
+// static bits (static_base), the static stencil cells are prepared only when
+// the previous mask may lack them, and every plane task runs in one launch.
 int bound_forcing(tdlb_cu* cu, int momsrc, bool prepare_static) {
-    std::vector<ForceSegment> segs(2);
-    segs[0].set = &cu->static_set;
-    segs[0].n_points = cu->n_static;
-    for (size_t l = 0; l + 1 < cu->static_levels.size(); l++)
-        segs[0].levels.push_back({cu->static_levels[l], cu->static_levels[l + 1]});
-    segs[1].direct_cache = cu->rot_cache;
-    segs[1].cache_base = cu->n_static;
-    segs[1].n_points = cu->n_rot;
-    for (size_t l = 0; l + 1 < cu->rot_levels.size(); l++)
-        segs[1].levels.push_back({cu->rot_levels[l], cu->rot_levels[l + 1]});
-    std::vector<std::pair<int64_t, int64_t>> prep;
-    if (prepare_static) prep.push_back({0, cu->n_static});
-    prep.push_back({cu->n_static, cu->n_static + cu->n_rot});
-    CHK(forcing_pass(cu, cu->pts, segs, prep, cu->static_base, momsrc));
+    uint64_t* prev = cu->fmask[cu->fm];
+    uint64_t* next = cu->fmask[1 - cu->fm];
+    const bool stored = momsrc == MOMSRC_STORED;
+    const void* Qsrc = stored ? cu->eq() : cu->Q[cu->cur];
+    const int qp = stored ? cu->eprec() : cu->prec();
+    HIPCHK(launch_copy(next, cu->static_base, cu->mask_bytes(), cu->stream));
+    if (prepare_static)
+        HIPCHK(launch_force_prepare(cu->eprec(), cu->pts.d, 0, cu->n_static, cu->F, prev, next, cu->g, cu->stream));
+    HIPCHK(launch_force_planes(qp, cu->dlay(), cu->prm.streaming, momsrc, cu->even ? 1 : 0, cu->fgeom(), Qsrc, cu->F,
+                               prev, next, cu->g, cu->alfa, cu->beta, cu->rho0, cu->stream));
+    if (cu->o_dirty) {
+        HIPCHK(launch_zero_interior_o(cu->O, cu->g, cu->stream));
+        cu->o_dirty = false;
+    }
+    cu->fm = 1 - cu->fm;
     cu->static_in_prev = true;
     return TDLB_OK;
 }
@@ -744,7 +1048,7 @@ int tdlb_cu_synchronize(tdlb_cu_t cu) {
     if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
     CHK(set_device(cu));
     HIPCHK(hipStreamSynchronize(cu->stream));
-    return TDLB_OK;
+    return check_team_barriers(cu);
 }
 
 int tdlb_cu_stream(tdlb_cu_t cu, void** s) {
@@ -887,23 +1191,15 @@ int tdlb_cu_bind_rushton(tdlb_cu_t cu, const tdlb_rushton_params* p, uint64_t st
     CHK(check_points(cu, stat.data(), (int64_t)stat.size()));
     CHK(check_points(cu, rot.data(), (int64_t)rot.size()));
 
-    std::vector<int64_t> order, offs;
-    compute_levels(cu, stat.data(), (int64_t)stat.size(), order, offs);
-    HostPoints hs;
-    to_host_points(cu, stat.data(), order, hs);
     cu->n_static = (int64_t)stat.size();
     cu->n_rot = (int64_t)rot.size();
-    cu->static_levels = offs;
-    cu->rot_levels.clear();
-    int64_t acc = cu->n_static;
-    cu->rot_levels.push_back(acc);
-    for (int c = 0; c < 5 * be.n_bcol; c++) {
-        acc += blade_class_size(be, c);
-        cu->rot_levels.push_back(acc);
-    }
+    cu->n_blade_classes = cu->n_rot > 0 ? 5 * be.n_bcol : 0;
+    std::vector<int64_t> order;
+    CHK(build_plane_set(cu, stat, be, p->blade_plane_lo + cu->prm.ghost, order, cu->static_set));
+    HostPoints hs;
+    to_host_points(cu, stat.data(), order, hs);
     CHK(cu->pts.reserve(cu->n_static + cu->n_rot, cu->tsize));
     CHK(upload_points(cu, cu->pts, 0, hs));
-    CHK(build_indexed_set(cu, stat.data(), order, cu->static_set));
     if (cu->rot_cache) (void)hipFree(cu->rot_cache);
     cu->rot_cache = nullptr;
     if (cu->n_rot > 0 && hipMalloc(&cu->rot_cache, (size_t)cu->n_rot * 9 * 4 * cu->tsize) != hipSuccess)
@@ -948,6 +1244,8 @@ int tdlb_cu_bind_rushton(tdlb_cu_t cu, const tdlb_rushton_params* p, uint64_t st
     cu->bound = true;
     cu->alfa = alfa;
     cu->beta = beta;
+    cu->xcc_rr = pipe_device_ok(cu, false);
+    CHK(setup_pipe(cu));
 
     // initial forcing of main.cpp:305 on the current density and momentum
     CHK(gen_blades(cu, angle, delta));
@@ -966,6 +1264,16 @@ int tdlb_cu_run(tdlb_cu_t cu, int64_t nsteps, uint64_t* step, double* angle) {
         const double delta = to_prec(cu, R.step_increment(*step));
         *angle += delta;                                        // running.angle += delta
         CHK(gen_blades(cu, *angle, delta));                     // updateRotatingGeometry
+        if (cu->pipe && cu->phase == PH_LAZY) {
+            const int c0 = prof_mark(cu);
+            CHK(pipe_step(cu));
+            const int c1 = prof_mark(cu);
+            if (c0 >= 0 && c1 >= 0) {
+                cu->coll.push_back({c0, c1});
+                cu->force.push_back({c1, c1});
+            }
+            continue;
+        }
         const int c0 = prof_mark(cu);
         if (cu->phase == PH_EXPLICIT) {
             void* q = cu->Q[cu->cur];
@@ -1263,13 +1571,38 @@ int tdlb_cu_vorticity(tdlb_cu_t cu, int32_t axis, int64_t cut, const uint8_t* ex
     return TDLB_OK;
 }
 
+int tdlb_cu_one_launch(tdlb_cu_t cu, int32_t* on) {
+    if (!cu || !on) return fail(TDLB_ERR_INVALID, "null argument");
+    *on = cu->bound && cu->pipe ? 1 : 0;
+    return TDLB_OK;
+}
+
+int tdlb_cu_one_launch_phases(tdlb_cu_t cu, double* us, int32_t n) {
+    if (!cu || !us || n < 3 * kXcds) return fail(TDLB_ERR_INVALID, "bad arguments");
+    if (!cu->pctr) return fail(TDLB_ERR_STATE, "no one-launch step set up");
+    CHK(set_device(cu));
+    HIPCHK(hipStreamSynchronize(cu->stream));
+    int khz = 0;
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, cu->device));
+    uint64_t t[kXcds][4];
+    for (int x = 0; x < kXcds; x++)
+        HIPCHK(hipMemcpy(t[x], cu->pctr + 64 * x + 8, sizeof(t[x]), hipMemcpyDeviceToHost));
+    uint64_t t0 = ~0ull;
+    for (int x = 0; x < kXcds; x++)
+        if (t[x][0]) t0 = std::min(t0, t[x][0]);
+    for (int x = 0; x < kXcds; x++)
+        for (int p = 0; p < 3; p++)
+            us[3 * x + p] = (t[x][p + 1] && t0 != ~0ull && khz > 0)
+                                ? (double)(int64_t)(t[x][p + 1] - t0) * 1e3 / (double)khz
+                                : -1.0;
+    return TDLB_OK;
+}
+
 int tdlb_cu_forcing_info(tdlb_cu_t cu, int64_t* n_static, int64_t* n_rotating, int64_t* n_levels) {
     if (!cu) return fail(TDLB_ERR_INVALID, "null handle");
     if (n_static) *n_static = cu->n_static;
     if (n_rotating) *n_rotating = cu->n_rot;
-    if (n_levels)
-        *n_levels = (int64_t)(cu->static_levels.empty() ? 0 : cu->static_levels.size() - 1) +
-                    (int64_t)(cu->rot_levels.empty() ? 0 : cu->rot_levels.size() - 1);
+    if (n_levels) *n_levels = (int64_t)cu->static_set.max_levels + cu->n_blade_classes;
     return TDLB_OK;
 }
 

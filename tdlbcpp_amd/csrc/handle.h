@@ -90,19 +90,37 @@ inline void set_q_geometry(Grid& g, int dlay, size_t ssize) {
 
 // Stencil cells of a point list with a moments cache: the sorted unique cells,
 // and for every point the 9 positions of its stencil cells in that list.
+// A bound tank's static set also carries its plane tasks (PlaneDesc): cells
+// grouped by plane, points ordered plane by plane and level by level.
 struct IndexedSet {
     uint32_t* cells = nullptr;
     int32_t* sidx = nullptr;
     void* cache = nullptr;
+    PlaneDesc* planes = nullptr;
+    int64_t* lev_off = nullptr;
+    PlaneBlock* blocks = nullptr;
+    uint32_t* bar = nullptr;
     int64_t n_cells = 0, n_points = 0;
+    int32_t n_planes = 0, n_blocks = 0, n_teams = 0, max_levels = 0;
+    std::vector<float> plane_cost;  // host: duration estimate of each plane's workgroups (planes order)
+    std::vector<PlaneBlock> block_list;  // host copy of blocks
+    std::vector<PlaneDesc> plane_list;   // host copy of planes
     void release() {
-        if (cells) (void)hipFree(cells);
-        if (sidx) (void)hipFree(sidx);
-        if (cache) (void)hipFree(cache);
+        void* ptrs[] = {cells, sidx, cache, planes, lev_off, blocks, bar};
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
         cells = nullptr;
         sidx = nullptr;
         cache = nullptr;
+        planes = nullptr;
+        lev_off = nullptr;
+        blocks = nullptr;
+        bar = nullptr;
         n_cells = n_points = 0;
+        n_planes = n_blocks = n_teams = max_levels = 0;
+        plane_cost.clear();
+        block_list.clear();
+        plane_list.clear();
     }
 };
 
@@ -165,10 +183,15 @@ struct tdlb_cu {
     IndexedSet static_set;               // stencil cells of the static points
     void* rot_cache = nullptr;           // 9 moment entries per blade point
     int64_t n_static = 0, n_rot = 0;
-    std::vector<int64_t> static_levels;  // CSR offsets into [0, n_static)
-    std::vector<int64_t> rot_levels;     // CSR offsets into [n_static, n_static + n_rot)
+    int32_t n_blade_classes = 0;         // blade colour classes (one forcing level each)
     BladeLayout blade{};
     double alfa = 0.97, beta = 1.9;
+
+    // one-launch Esotwist step (launch_eso_pipe), set up at bind when the tank allows it
+    bool pipe = false;
+    bool xcc_rr = false;  // launches dispatch workgroup b to XCD b mod kXcds (checked at bind)
+    int32_t* rowcell = nullptr;
+    uint32_t* pctr = nullptr;
 
     DevPoints api_pts;  // points of the generic forcing call
     IndexedSet api_set;
@@ -204,6 +227,27 @@ struct tdlb_cu {
     int prec() const { return prm.precision; }            // kernels on the stored populations
     int eprec() const { return prm.precision == TDLB_PREC_F64 ? 1 : 0; }  // kernels on the explicit Q
     void* eq() const { return shifted() ? M : Q[cur]; }   // the explicit Q array
+
+    // the forcing planes of the bound tank
+    ForceGeom fgeom() const {
+        ForceGeom f{};
+        f.pts = pts.d;
+        f.n_static = n_static;
+        f.n_rot = n_rot;
+        f.cells = static_set.cells;
+        f.sidx = static_set.sidx;
+        f.scache = static_set.cache;
+        f.rcache = rot_cache;
+        f.planes = static_set.planes;
+        f.n_planes = static_set.n_planes;
+        f.lev_off = static_set.lev_off;
+        f.blocks = static_set.blocks;
+        f.n_blocks = static_set.n_blocks;
+        f.bar = static_set.bar;
+        f.local_teams = xcc_rr ? 1 : 0;
+        f.blade = blade;
+        return f;
+    }
 
     CollideArgs cargs() const {
         CollideArgs c;

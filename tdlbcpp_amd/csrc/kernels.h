@@ -112,6 +112,90 @@ hipError_t launch_ib_moments_points(int prec, int layout, int streaming, int mom
 hipError_t launch_force_level_cached(int prec, const PointsDev& p, int64_t begin, int64_t count,
                                      const int32_t* sidx, int64_t cache_base, const void* cache, void* F,
                                      const Grid& g, double alfa, double beta, hipStream_t s);
+// ---- IB forcing of a bound tank by j planes.  Stencils span i and k at the
+// point's j, so points of different planes never share a cell and each plane
+// is an independent task: a workgroup (or a team of workgroups for a plane
+// with many points, such as the impeller disk's) runs a plane's blade
+// occupancy bits, the density and momentum of its stencil cells, then its
+// static levels and blade colour classes in list order, with a barrier
+// between levels instead of a kernel launch.
+struct PlaneDesc {
+    int32_t j;                     // plane index including the ghost offset
+    int32_t n_levels;              // static levels of the plane
+    int64_t cell_begin, cell_end;  // its unique static stencil cells in ForceGeom::cells / scache
+    int64_t lev;                   // ForceGeom::lev_off[lev .. lev + n_levels]: point offsets of its levels
+    int32_t team;                  // workgroups running the plane together
+    int32_t bar;                   // its barrier in ForceGeom::bar (team > 1), else -1
+};
+// workgroup -> (plane, rank in the plane's team); plane -1: an empty slot
+struct PlaneBlock {
+    int32_t plane, rank;
+};
+struct ForceGeom {
+    PointsDev pts;           // static points [0, n_static) plane by plane, level by level; blades after
+    int64_t n_static, n_rot;
+    const uint32_t* cells;   // unique static stencil cells, grouped by plane
+    const int32_t* sidx;     // per static point: the 9 positions of its stencil cells in cells
+    void* scache;            // {rho, jx, jy, jz} in T per entry of cells
+    void* rcache;            // {rho, jx, jy, jz} in T x 9 per blade point
+    const PlaneDesc* planes;
+    int32_t n_planes;
+    const PlaneBlock* blocks;  // team workgroups first, then the longest tasks first
+    int32_t n_blocks;
+    const int64_t* lev_off;
+    uint32_t* bar;           // kBarWords words per team: arrivals, generation, flags, XCC ids seen
+    BladeLayout blade;       // n_blades 0: no blades
+    int32_t local_teams;     // 1: a team's workgroups sit kXcds apart in blocks, so one XCD runs them
+                             // (launches deal workgroups to the XCDs in turn)
+};
+constexpr int kBarWords = 4;
+// points one pass of a plane task handles: 16 wavefronts of 7 groups of 9 lanes
+constexpr int kForcePlanePoints = 112;
+// One forcing pass of a bound tank (every plane of fg): next must already hold
+// the static occupancy bits; blade stencil cells missing from prev get F = 0
+// and their bit in next.
+hipError_t launch_force_planes(int prec, int layout, int streaming, int momsrc, int even, const ForceGeom& fg,
+                               const void* Q, void* F, const uint64_t* prev, uint64_t* next, const Grid& g,
+                               double alfa, double beta, float rho0, hipStream_t s);
+
+// ---- One Esotwist main-loop step in one persistent launch (LIJK, float
+// arithmetic): the fused stream-collide of every row, the forcing gather of
+// each row soon after the rows it reads are written (lines still in L2 or the
+// memory-side cache), then the forcing levels of every plane.  Each XCD takes
+// a contiguous range of j planes from its own work queue (the hardware XCC id
+// picks the queue), so the rows a gather reads were written through the same
+// L2; only the two edge planes of a range wait for the neighbouring XCDs.
+constexpr int kXcds = 8;
+struct PipeArgs {
+    void* Q;                      // the lattice (stored type)
+    void* F;                      // float
+    void* Nu;                     // stored Nu type
+    const uint64_t* prev;         // occupancy read by this step's collision
+    uint64_t* next;               // occupancy after this step's forcing
+    const uint64_t* static_base;  // next starts from these bits
+    ForceGeom fg;
+    const int32_t* rowcell;       // fg.cells index of the first static stencil cell of row (i, j):
+                                  // rowcell[j * (xg1 + 2) + i], i = 1 .. xg1 + 1 (end of the plane)
+    uint32_t* ctr;                // pipe_counter_words(g) words, zero before the launch
+};
+inline size_t pipe_counter_words(const Grid& g) { return (size_t)kXcds * (64 + (size_t)g.xg1 + 2); }
+// planes [jlo, jhi] of XCD x: the interior planes split into kXcds contiguous ranges
+TDLB_HD inline void pipe_range(int yg1, int x, int& jlo, int& jhi) {
+    const int base = yg1 / kXcds, rem = yg1 % kXcds;
+    jlo = 1 + x * base + (x < rem ? x : rem);
+    jhi = jlo + base + (x < rem ? 1 : 0) - 1;
+}
+// prec 0, 2, 3, 4 (float arithmetic); hint: the streaming cache hint of the
+// fused kernel (line_rows); blocks: the persistent grid
+hipError_t launch_eso_pipe(int prec, int even, int hint, const PipeArgs& a, const Grid& g, const CollideArgs& c,
+                           float rho0, double alfa, double beta, uint32_t blocks, uint32_t lanes, hipStream_t s);
+// lanes per row block of the stream-collide kernels for this grid (the pipe's block size)
+uint32_t row_lanes(const Grid& g);
+// every k-row of the population planes (stored values of ssize bytes) is whole 128 B lines
+bool line_rows_bytes(const Grid& g, size_t ssize);
+// 1 when workgroups of a launch see XCC ids 0 .. kXcds - 1 (each XCD's queue has workers)
+int pipe_xcc_check(hipStream_t s);
+
 // FP64 sums over the interior cells: {rho, jx, jy, jz, Fx, Fy, Fz, torque}.  The
 // torque sum x*Fz - z*Fx runs over cells whose centre (x, z) relative to the
 // axis (cx, cz) lies within r2 (squared radius; negative: none).
@@ -171,8 +255,10 @@ hipError_t launch_solid_mask(double cx, double cz, double radius, uint8_t* out, 
 // step is a fixed launch sequence that can be captured in a HIP graph.
 enum Tile : int32_t {
     TILE_ROW64 = 0,  // a wavefront takes 64 consecutive k of one (i, j) row
-    TILE_PLANE = 1,  // a block takes 256 consecutive cells of one i plane (ghost cells idle)
+    TILE_PLANE = 1,  // a block takes consecutive cells of one i plane (ghost cells idle)
+    TILE_FLAT = 2,   // a block takes consecutive interior cells of the tank, rows back to back
 };
+constexpr int kTiles = 3;
 
 struct MemberDev {
     Grid g;
@@ -185,14 +271,7 @@ struct MemberDev {
     void* Nu;  // in the member's Nu storage type (float, or its 16-bit format)
     uint64_t* fmask[2];
     const uint64_t* static_base;
-    PointsDev pts;  // static points [0, n_static), blades [n_static, n_static + n_rot)
-    int64_t n_static, n_rot;
-    const uint32_t* scells;  // unique stencil cells of the static points
-    const int32_t* sidx;     // 9 per static point into scells / scache
-    int64_t n_scells;
-    void* scache;  // Mom4<float> per unique static cell
-    void* rcache;  // Mom4<float> x 9 per blade point
-    BladeLayout blade;
+    ForceGeom fg;            // IB points, stencil caches and plane tasks of the member
     const BladeStep* steps;  // blade orientation of step done + 1, done + 2, ...
     int64_t n_steps;         // steps this member may still take in the batch
     // advanced on the device after every step
@@ -200,38 +279,32 @@ struct MemberDev {
     int64_t done;
 };
 
-// points [begin, begin + count) of member `member` in one forcing level
-struct LevelSeg {
-    int32_t member, pad;
-    int64_t begin, count;
+// one workgroup of the batch's forcing launch: member and its PlaneBlock
+struct PlaneTask {
+    int32_t member, block;
 };
 
 // Device tables of one batch step.  off_* are member prefix sums (n_members + 1
-// entries) of blocks / threads; level l of the static (blade) lists is
-// segs[lv_*[l] .. lv_*[l + 1]) with thread prefix sums in seg_cum.
+// entries) of blocks / threads; tasks: every plane of every member, heaviest
+// first.
 struct BatchPlan {
     const MemberDev* mem;
     MemberDev* mem_rw;
     int32_t n_members;
-    const int64_t* off_coll;    // blocks of the fused kernel
-    const int64_t* off_rot;     // blade points
-    const int64_t* off_words;   // occupancy-mask words
-    const int64_t* off_scells;  // unique static stencil cells
-    const LevelSeg* segs;
-    const int64_t* seg_cum;
-    int64_t tot_coll, tot_rot, tot_words, tot_scells;
-    // host copies of the level structure (launch sizes)
-    int32_t n_slevels, n_rlevels;
-    const int64_t* lv_static;   // host: n_slevels + 1 segment offsets
-    const int64_t* lv_blade;    // host: n_rlevels + 1 segment offsets
-    const int64_t* lv_threads;  // host: threads of static levels then blade levels
+    const int64_t* off_coll;   // blocks of the fused kernel
+    const int64_t* off_rot;    // blade points
+    const int64_t* off_words;  // occupancy-mask words
+    const PlaneTask* tasks;
+    int64_t tot_coll, tot_rot, tot_words, n_tasks;
+    int32_t bs;  // threads per block of the fused kernel (256 or 512)
 };
 
-// Enqueue one main-loop step of every active member (fused stream-collide,
-// blade generation, occupancy masks, moment caches, forcing levels, advance).
-hipError_t launch_batch_step(int prec, int layout, int streaming, const BatchPlan& plan, hipStream_t s);
-// blocks of the fused kernel for one member with the given tile
-int64_t batch_coll_blocks(const Grid& g, int tile);
+// Enqueue `steps` main-loop steps of every active member: per step blade
+// generation, the fused stream-collide, the static occupancy bits, the forcing
+// planes, advance.
+hipError_t launch_batch_step(int prec, int layout, int streaming, const BatchPlan& plan, int steps, hipStream_t s);
+// blocks of the fused kernel for one member with the given tile and block size
+int64_t batch_coll_blocks(const Grid& g, int tile, int bs);
 // The fused kernel alone over every member (autotuning; advances no state).
 hipError_t launch_batch_fused(int prec, int layout, int streaming, const BatchPlan& plan, hipStream_t s);
 

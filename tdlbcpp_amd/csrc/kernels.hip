@@ -13,6 +13,7 @@
 //   forcing             Forcing.hpp:56-176
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "codec16.h"
@@ -46,10 +47,10 @@ inline uint32_t blocks_for(uint64_t n, int bs = kBlock) {
 // neighbours', which neighbouring rows then find in the same L2 (measured 13-20 %
 // faster); in LIJK every line is read by one row only and the default order,
 // all XCDs streaming through one region of HBM together, is faster (5 %).
-constexpr uint32_t kXcds = 8;
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
-    const uint32_t x = b % kXcds, q = b / kXcds;
-    const uint32_t per = n / kXcds, rem = n % kXcds;
+    constexpr uint32_t nx = kXcds;
+    const uint32_t x = b % nx, q = b / nx;
+    const uint32_t per = n / nx, rem = n % nx;
     return x * per + min(x, rem) + q;
 }
 template <int L>
@@ -880,23 +881,44 @@ __global__ __launch_bounds__(kBlock) void k_ib_moments_points(PointsDev p, int64
 constexpr int kGroup = 9;                  // lanes per point
 constexpr int kGroupsPerWave = 64 / kGroup;  // 7 points per wavefront, lane 63 idle
 
-template <typename T, typename D, bool INDEXED>
-__device__ __forceinline__ void force_point9(const PointsDev& p, int64_t q, bool valid, int s, int base,
-                                             const int32_t* __restrict__ sidx, int64_t cache_base,
-                                             const Mom4<T>* __restrict__ cache, T* __restrict__ F, const Grid& g,
-                                             T alfa, T beta) {
+// The inputs of stencil cell s of point q, as loaded (force_point9 in two
+// parts, so that a pass can load the next pass's points while it computes).
+template <typename T>
+struct PointRaw {
+    int pi, pj, pk;
+    T ifr, kfr, u, v, w;
+    int64_t e;  // cache entry
+};
+template <typename T, bool INDEXED>
+__device__ __forceinline__ PointRaw<T> point_raw(const PointsDev& p, int64_t q, bool valid, int s,
+                                                 const int32_t* __restrict__ sidx, int64_t cache_base) {
+    PointRaw<T> r{};
+    if (valid) {
+        r.pi = p.i[q];
+        r.pj = p.j[q];
+        r.pk = p.k[q];
+        r.ifr = static_cast<const T*>(p.ifrac)[q];
+        r.kfr = static_cast<const T*>(p.kfrac)[q];
+        r.u = static_cast<const T*>(p.u)[q];
+        r.v = static_cast<const T*>(p.v)[q];
+        r.w = static_cast<const T*>(p.w)[q];
+        r.e = INDEXED ? (int64_t)sidx[9 * q + s] : 9 * (q - cache_base) + s;
+    }
+    return r;
+}
+
+template <typename T, typename D>
+__device__ __forceinline__ void force_point9_raw(const PointRaw<T>& r, bool valid, int s, int base,
+                                                 const Mom4<T>* __restrict__ cache, T* __restrict__ F, const Grid& g,
+                                                 T alfa, T beta) {
     T w = T(0), tr = T(0), tx = T(0), ty = T(0), tz = T(0), fo0 = T(0), fo1 = T(0), fo2 = T(0);
-    T u = T(0), v = T(0), ww = T(0);
     uint64_t cell = 0;
     if (valid) {
-        const int pi = p.i[q], pj = p.j[q], pk = p.k[q];
-        const T ifr = static_cast<const T*>(p.ifrac)[q], kfr = static_cast<const T*>(p.kfrac)[q];
         const int a = s % 3, b = s / 3;
-        w = ib_weight<T>(-ifr + T(a - 1)) * ib_weight<T>(-kfr + T(b - 1));
-        const int i2 = wrap_ik(pi + a - 1, g.xg1), k2 = wrap_ik(pk + b - 1, g.zg1);
-        cell = cell_index(g, i2, pj, k2);
-        const int64_t e = INDEXED ? (int64_t)sidx[9 * q + s] : 9 * (q - cache_base) + s;
-        const Mom4<T> m = cache[e];
+        w = ib_weight<T>(-r.ifr + T(a - 1)) * ib_weight<T>(-r.kfr + T(b - 1));
+        const int i2 = wrap_ik(r.pi + a - 1, g.xg1), k2 = wrap_ik(r.pk + b - 1, g.zg1);
+        cell = cell_index(g, i2, r.pj, k2);
+        const Mom4<T> m = cache[r.e];
         fo0 = F[3 * cell + 0];
         fo1 = F[3 * cell + 1];
         fo2 = F[3 * cell + 2];
@@ -907,9 +929,6 @@ __device__ __forceinline__ void force_point9(const PointsDev& p, int64_t q, bool
         tx = w * xx;
         ty = w * yy;
         tz = w * zz;
-        u = static_cast<const T*>(p.u)[q];
-        v = static_cast<const T*>(p.v)[q];
-        ww = static_cast<const T*>(p.w)[q];
     }
     T rhoSum = T(0.0), xSum = T(0.0), ySum = T(0.0), zSum = T(0.0);
 #pragma unroll
@@ -920,13 +939,22 @@ __device__ __forceinline__ void force_point9(const PointsDev& p, int64_t q, bool
         zSum += __shfl(tz, base + t, 64);
     }
     if (!valid) return;
-    xSum -= rhoSum * u;
-    ySum -= rhoSum * v;
-    zSum -= rhoSum * ww;
+    xSum -= rhoSum * r.u;
+    ySum -= rhoSum * r.v;
+    zSum -= rhoSum * r.w;
     const T bp = beta * w;
     F[3 * cell + 0] = alfa * fo0 - bp * xSum;
     F[3 * cell + 1] = alfa * fo1 - bp * ySum;
     F[3 * cell + 2] = alfa * fo2 - bp * zSum;
+}
+
+template <typename T, typename D, bool INDEXED>
+__device__ __forceinline__ void force_point9(const PointsDev& p, int64_t q, bool valid, int s, int base,
+                                             const int32_t* __restrict__ sidx, int64_t cache_base,
+                                             const Mom4<T>* __restrict__ cache, T* __restrict__ F, const Grid& g,
+                                             T alfa, T beta) {
+    const PointRaw<T> r = point_raw<T, INDEXED>(p, q, valid, s, sidx, cache_base);
+    force_point9_raw<T, D>(r, valid, s, base, cache, F, g, alfa, beta);
 }
 
 // lane -> (point index within the launch, stencil cell, group base lane)
@@ -954,6 +982,511 @@ __global__ __launch_bounds__(kBlock) void k_force_level_cached(PointsDev p, int6
     const bool valid = t >= 0 && t < count;
     force_point9<T, D, INDEXED>(p, begin + (valid ? t : 0), valid, s, base, sidx, cache_base, cache, F, g, alfa,
                                 beta);
+}
+
+// ------------------------------------------------------------------ IB forcing by planes
+// A bound tank's forcing pass as independent plane tasks (ForceGeom in
+// kernels.h).  A task first clears F and sets the occupancy bit of the blade
+// stencil cells missing from the previous mask (prepare_point) and computes the
+// density and momentum of its static and blade stencil cells; then it runs its
+// static levels and its blade colour classes in order, passes of up to
+// kPlanePoints points per workgroup, 9 lanes per point (force_point9), with a
+// barrier after every level: the sequential list order of Forcing.hpp:73-135
+// within the plane, without a kernel launch per level.  A plane with many
+// points runs on a team of workgroups that meet at a barrier in global memory
+// (team_barrier); teams are the first workgroups of the launch, so they are
+// dispatched, and resident, before any other workgroup of it.
+constexpr int kPlaneBlock = 1024;
+constexpr int kPlanePoints = (kPlaneBlock / 64) * kGroupsPerWave;
+static_assert(kPlanePoints == kForcePlanePoints, "points per pass of a plane task");
+// polls of a team barrier before it gives up and flags a time-out (about a
+// second; every member of a team is resident, so a healthy barrier takes
+// microseconds)
+constexpr uint32_t kBarSpinLimit = 1u << 20;
+
+// Sense-reversing barrier of a team of workgroups: bar[0] counts arrivals,
+// bar[1] is the generation, bar[2] the time-out flag.  The last arrival resets
+// the count (waiting for that reset to complete) and then advances the
+// generation, so the barrier is ready for the next level and the next launch.
+// Local teams (every member on one XCD, ForceGeom::local_teams) exchange F and
+// the caches through that XCD's L2: each wave's stores complete before the
+// arrival, and each CU drops its L1 after the wait; the last arrival checks
+// that every member reported the same XCC id (bar[2] bit 2 otherwise).  Other teams fence at agent
+// scope (L2 write-back before the arrival, invalidation after the wait).  The
+// wait polls with relaxed loads (an acquire load would invalidate the L2 on
+// every poll).
+__device__ __forceinline__ void team_barrier(uint32_t* bar, int team, bool local) {
+    if (local) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (!local) __threadfence();
+        if (local) {
+            // bar[3] collects the XCC ids of the arrivals: one XCD must run the team
+            uint32_t x;
+            asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(x));
+            (void)__hip_atomic_fetch_or(&bar[3], 1u << (x & 15u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        }
+        const uint32_t gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == (uint32_t)team - 1) {
+            if (local) {
+                const uint32_t seen = __hip_atomic_exchange(&bar[3], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (__builtin_popcount(seen) != 1) atomicOr(&bar[2], 2u);
+            }
+            __hip_atomic_exchange(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                if (++spins >= kBarSpinLimit) {
+                    atomicOr(&bar[2], 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        if (!local) __threadfence();
+    }
+    __syncthreads();
+    if (local) asm volatile("buffer_inv sc0" ::: "memory");
+}
+
+// lane -> point slot of a pass (wave-major, 7 groups of 9 lanes per wave), -1 for lane 63
+__device__ __forceinline__ int plane_slot(int& s, int& base) {
+    const int lane = (int)(threadIdx.x & 63), grp = lane / kGroup;
+    s = lane - grp * kGroup;
+    base = grp * kGroup;
+    return grp < kGroupsPerWave ? (int)(threadIdx.x >> 6) * kGroupsPerWave + grp : -1;
+}
+
+// Blade colour class c of the generator's colour-major list (rushton.h
+// blade_decode): its first list index, blades and radial points; point
+// (blade bi, plane bp, radial ri) of the class is at base + (bi * planes + bp) * nr + ri.
+struct BladeClass {
+    int64_t base;
+    int nb, nr;
+};
+__device__ __forceinline__ BladeEnum blade_enum_of(const BladeLayout& bl) {
+    BladeEnum e;
+    e.n_blades = bl.n_blades;
+    e.n_planes = bl.n_planes;
+    e.n_radial = bl.n_radial;
+    e.n_bcol = (bl.n_blades % 2 == 0) ? 2 : 3;
+    return e;
+}
+__device__ __forceinline__ BladeClass blade_class(const BladeEnum& e, int c, int64_t base) {
+    const int bcol = c / 5, rmod = c % 5;
+    BladeClass bc;
+    bc.base = base;
+    bc.nb = e.n_blades > bcol ? (e.n_blades - bcol + e.n_bcol - 1) / e.n_bcol : 0;
+    bc.nr = e.n_radial > rmod ? (e.n_radial - rmod + 4) / 5 : 0;
+    return bc;
+}
+// blade plane index of plane j (ghost included) and its colour classes (0: no blades)
+__device__ __forceinline__ int blade_plane(const ForceGeom& fg, int j, int& ncls) {
+    const BladeLayout& bl = fg.blade;
+    const int bp = j - bl.ghost - bl.j_lo;
+    ncls = (bl.n_blades > 0 && fg.n_rot > 0 && bp >= 0 && bp < bl.n_planes) ? 5 * blade_enum_of(bl).n_bcol : 0;
+    return bp;
+}
+
+// Static stencil cells [c0, c1) of fg.cells: density and momentum into the cache,
+// thread t of `threads` taking c0 + t, c0 + t + threads, ...
+template <typename PR, int L, int STREAM, int MOMSRC, int EVEN>
+__device__ __forceinline__ void gather_cells(const ForceGeom& fg, int64_t c0, int64_t c1, int64_t t,
+                                             int64_t threads, const typename PR::S* __restrict__ Q, const Grid& g,
+                                             float rho0) {
+    auto* scache = static_cast<Mom4<typename PR::T>*>(fg.scache);
+    for (int64_t e = c0 + t; e < c1; e += threads) {
+        int i, j, k;
+        decode_cell(g, fg.cells[e], i, j, k);
+        Mom4<typename PR::T> m;
+        cell_rho_j<PR, L, STREAM, MOMSRC, EVEN>(Q, g, i, j, k, m.r, m.x, m.y, m.z, rho0);
+        scache[e] = m;
+    }
+}
+
+// Blade stencil cells of plane j: occupancy bit in next, F = 0 where the
+// previous mask lacks the cell (prepare_point), density and momentum into the
+// blade cache.
+template <typename PR, int L, int STREAM, int MOMSRC, int EVEN>
+__device__ void gather_blades(const ForceGeom& fg, int j, int64_t t0, int64_t threads,
+                              const typename PR::S* __restrict__ Q, typename PR::T* __restrict__ F,
+                              const uint64_t* __restrict__ prev, uint64_t* __restrict__ next, const Grid& g,
+                              float rho0) {
+    using T = typename PR::T;
+    auto* rcache = static_cast<Mom4<T>*>(fg.rcache);
+    int ncls;
+    const int bp = blade_plane(fg, j, ncls);
+    const BladeEnum be = blade_enum_of(fg.blade);
+    int64_t base = 0;
+    for (int c = 0; c < ncls; c++) {
+        const BladeClass bc = blade_class(be, c, base);
+        const int64_t n9 = 9 * (int64_t)bc.nb * bc.nr;
+        for (int64_t t = t0; t < n9; t += threads) {
+            const int64_t pt = t / 9;
+            const int s = (int)(t - 9 * pt);
+            const int bi = (int)(pt / bc.nr), ri = (int)(pt - (int64_t)bi * bc.nr);
+            const int64_t e = bc.base + ((int64_t)bi * fg.blade.n_planes + bp) * bc.nr + ri;
+            const int64_t q = fg.n_static + e;
+            const int pj = fg.pts.j[q];
+            const int i2 = wrap_ik(fg.pts.i[q] + (s % 3) - 1, g.xg1), k2 = wrap_ik(fg.pts.k[q] + (s / 3) - 1, g.zg1);
+            if (!mask_test(prev, g, i2, pj, k2)) {
+                const uint64_t cc = cell_index(g, i2, pj, k2);
+                F[3 * cc + 0] = T(0);
+                F[3 * cc + 1] = T(0);
+                F[3 * cc + 2] = T(0);
+            }
+            atomicOr((unsigned long long*)&next[mask_word_index(g, i2, pj, k2)], 1ull << (k2 & 63));
+            Mom4<T> m;
+            cell_rho_j<PR, L, STREAM, MOMSRC, EVEN>(Q, g, i2, pj, k2, m.r, m.x, m.y, m.z, rho0);
+            rcache[9 * e + s] = m;
+        }
+        base += blade_class_size(be, c);
+    }
+}
+
+// The plane's static levels, then its blade colour classes, by workgroup
+// `rank` of a team of `team` (passes of (blockDim.x / 64) * 7 points), with
+// sync() after every level.
+template <typename T, typename D, typename Sync>
+__device__ void plane_levels(const ForceGeom& fg, const PlaneDesc& pd, int rank, int team, T* __restrict__ F,
+                             const Grid& g, T alfa, T beta, Sync&& sync) {
+    const auto* scache = static_cast<const Mom4<T>*>(fg.scache);
+    const auto* rcache = static_cast<const Mom4<T>*>(fg.rcache);
+    int s, gbase;
+    const int slot = plane_slot(s, gbase);
+    const int64_t per = (int64_t)(blockDim.x >> 6) * kGroupsPerWave;
+    const int64_t p0 = (int64_t)rank * per, pstride = (int64_t)team * per;
+    // Each pass loads the next pass's points (positions, fractions, velocities,
+    // cache entries) before it computes: the point lists do not change during
+    // the pass.  Every static level ends with one sync(), also a level this
+    // workgroup has no pass in.
+    const int nlev = pd.n_levels;
+    auto lev_end = [&](int lv) { return fg.lev_off[pd.lev + lv + 1]; };
+    auto first = [&](int lv) { return fg.lev_off[pd.lev + lv] + p0; };
+    auto raw_at = [&](int lv, int64_t qq) {
+        const bool ok = lv < nlev && slot >= 0 && qq + slot < lev_end(lv);
+        return point_raw<T, true>(fg.pts, ok ? qq + slot : 0, ok, s, fg.sidx, 0);
+    };
+    int l = 0;
+    while (l < nlev && first(l) >= lev_end(l)) {
+        sync();
+        l++;
+    }
+    int64_t q = l < nlev ? first(l) : 0;
+    PointRaw<T> cur = raw_at(l, q);
+    while (l < nlev) {
+        // the next pass of this workgroup: later in level l, else the first of the next level that has one
+        int nl = l;
+        int64_t nq = q + pstride;
+        if (nq >= lev_end(l)) {
+            nl = l + 1;
+            while (nl < nlev && first(nl) >= lev_end(nl)) nl++;
+            nq = nl < nlev ? first(nl) : 0;
+        }
+        const PointRaw<T> nxt = raw_at(nl, nq);
+        force_point9_raw<T, D>(cur, slot >= 0 && q + slot < lev_end(l), s, gbase, scache, F, g, alfa, beta);
+        for (int k = l; k < nl; k++) sync();  // levels l .. nl - 1 are done
+        l = nl;
+        q = nq;
+        cur = nxt;
+    }
+    int ncls;
+    const int bp = blade_plane(fg, pd.j, ncls);
+    const BladeEnum be = blade_enum_of(fg.blade);
+    int64_t base = 0;
+    for (int c = 0; c < ncls; c++) {
+        const BladeClass bc = blade_class(be, c, base);
+        const int64_t n = (int64_t)bc.nb * bc.nr;
+        for (int64_t u0 = p0; u0 < n; u0 += pstride) {
+            const int64_t t = u0 + slot;
+            const bool valid = slot >= 0 && t < n;
+            int64_t qb = fg.n_static;
+            if (valid) {
+                const int bi = (int)(t / bc.nr), ri = (int)(t - (int64_t)bi * bc.nr);
+                qb += bc.base + ((int64_t)bi * fg.blade.n_planes + bp) * bc.nr + ri;
+            }
+            force_point9<T, D, false>(fg.pts, qb, valid, s, gbase, nullptr, fg.n_static, rcache, F, g, alfa, beta);
+        }
+        sync();
+        base += blade_class_size(be, c);
+    }
+}
+
+// workgroup `rank` of the plane's team
+template <typename PR, int L, int STREAM, int MOMSRC, int EVEN>
+__device__ void force_plane(const ForceGeom& fg, const PlaneDesc& pd, int rank,
+                            const typename PR::S* __restrict__ Q, typename PR::T* __restrict__ F,
+                            const uint64_t* __restrict__ prev, uint64_t* __restrict__ next, const Grid& g,
+                            typename PR::T alfa, typename PR::T beta, float rho0) {
+    const int team = pd.team;
+    uint32_t* bar = team > 1 ? fg.bar + kBarWords * pd.bar : nullptr;
+    const int64_t t0 = (int64_t)rank * blockDim.x + threadIdx.x, threads = (int64_t)team * blockDim.x;
+    const bool local = fg.local_teams != 0;
+    auto sync = [&]() {
+        if (team > 1) team_barrier(bar, team, local);
+        else __syncthreads();
+    };
+    gather_cells<PR, L, STREAM, MOMSRC, EVEN>(fg, pd.cell_begin, pd.cell_end, t0, threads, Q, g, rho0);
+    gather_blades<PR, L, STREAM, MOMSRC, EVEN>(fg, pd.j, t0, threads, Q, F, prev, next, g, rho0);
+    sync();
+    plane_levels<typename PR::T, typename PR::D>(fg, pd, rank, team, F, g, alfa, beta, sync);
+}
+
+template <typename PR, int L, int STREAM, int MOMSRC, int EVEN>
+__global__ __launch_bounds__(kPlaneBlock) void k_force_planes(ForceGeom fg, const typename PR::S* __restrict__ Q,
+                                                              typename PR::T* __restrict__ F,
+                                                              const uint64_t* __restrict__ prev,
+                                                              uint64_t* __restrict__ next, Grid g,
+                                                              typename PR::T alfa, typename PR::T beta, float rho0) {
+    const PlaneBlock pb = fg.blocks[blockIdx.x];
+    if (pb.plane < 0) return;
+    force_plane<PR, L, STREAM, MOMSRC, EVEN>(fg, fg.planes[pb.plane], pb.rank, Q, F, prev, next, g, alfa, beta,
+                                             rho0);
+}
+
+// The levels of every plane after the gathers of the one-launch step: a
+// workgroup (or team) per plane runs its blade stencil cells (occupancy, F,
+// moments) and then its static levels and blade classes.
+constexpr int kLevelBlock = 1024;
+template <typename PR, int EVEN>
+__global__ __launch_bounds__(kLevelBlock) void k_force_levels(ForceGeom fg, const typename PR::S* __restrict__ Q,
+                                                              float* __restrict__ F, const uint64_t* __restrict__ prev,
+                                                              uint64_t* __restrict__ next, Grid g, float alfa,
+                                                              float beta, float rho0) {
+    const PlaneBlock pb = fg.blocks[blockIdx.x];
+    if (pb.plane < 0) return;
+    const PlaneDesc& pd = fg.planes[pb.plane];
+    const int team = pd.team;
+    uint32_t* bar = team > 1 ? fg.bar + kBarWords * pd.bar : nullptr;
+    const bool local = fg.local_teams != 0;
+    auto sync = [&]() {
+        if (team > 1) team_barrier(bar, team, local);
+        else __syncthreads();
+    };
+    gather_blades<PR, LayoutLIJK, StreamEsotwist, MOMSRC_ESO_POPS, EVEN>(
+        fg, pd.j, (int64_t)pb.rank * blockDim.x + threadIdx.x, (int64_t)team * blockDim.x, Q, F, prev, next, g, rho0);
+    sync();
+    plane_levels<float, typename PR::D>(fg, pd, pb.rank, team, F, g, alfa, beta, sync);
+}
+
+// ------------------------------------------------------------------ pipelined Esotwist step
+// PipeArgs in kernels.h.  Work items of XCD x, in queue order (nJ planes
+// [jlo, jhi], R = nJ * nkb row blocks per i step):
+//   A  for i = 1 .. xg1: the R rows (i, j) of the range; after the rows of i
+//      the gather of i - 2 (interior planes jlo < j < jhi; a gather of i reads
+//      what the rows of i - 1 .. i + 1 wrote, i wrapping periodically), then
+//      the gathers of xg1 - 1, xg1 and 1
+//   B  the edge planes jlo and jhi (their gathers read rows of the
+//      neighbouring XCDs), in chunks, once those XCDs have released their rows
+//   C  the forcing levels of the range's planes (PlaneDesc), teams first
+// A row item also copies its rows' static occupancy words into next.  Items
+// wait only for earlier items, so every wait ends; spins are bounded anyway
+// (kBarSpinLimit, flag in the XCD's counters).
+//
+// Counters (uint32, zero at launch): XCD x at ctr[64 x]: +0 next item, +1 rows
+// done, +2 gathers done, +3 rows released to the other XCDs, +4 time-out flag;
+// rows done per i step at ctr[64 kXcds + x (xg1 + 2) + i].  The XCD-local
+// counters are relaxed agent-scope atomics: the rows a gather reads were
+// written through this XCD's L2, and each row's stores are complete
+// (s_waitcnt) before its count.  The release of a range for the neighbours is
+// an agent-scope fence pair.
+__device__ __forceinline__ uint32_t xcc_id() {
+    uint32_t x;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(x));
+    return x;
+}
+__device__ __forceinline__ uint32_t ld_ctr(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// thread 0 polls until every counter reaches its target (or the spin limit)
+__device__ __forceinline__ void pipe_wait(const uint32_t* a, uint32_t ta, const uint32_t* b, uint32_t tb,
+                                          const uint32_t* c, uint32_t tc, uint32_t* flag) {
+    if (threadIdx.x == 0) {
+        uint32_t spins = 0;
+        while (ld_ctr(a) < ta || (b && ld_ctr(b) < tb) || (c && ld_ctr(c) < tc)) {
+            if (++spins >= kBarSpinLimit) {
+                atomicOr(flag, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+}
+// every store of the workgroup is complete in L2, then thread 0 counts it
+__device__ __forceinline__ void pipe_done(uint32_t* ctr) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// rows of one item of the pipe (consecutive j of one i step)
+constexpr uint32_t kPipeRows = 8;
+
+// Occupancy of the pipe: 8 waves per SIMD for the 16-bit formats (61
+// registers); 5 for FP32, the most without spills (92 registers; the separate
+// FP32 kernel runs 6 at 74).
+template <typename PR>
+constexpr int pipe_waves() { return std::is_same<typename PR::S, float>::value ? 5 : 8; }
+
+template <typename PR, int EVEN, bool HINT>
+__global__ __launch_bounds__(kRowBlock) __attribute__((amdgpu_waves_per_eu(pipe_waves<PR>(), 8)))
+void k_eso_pipe(PipeArgs a, Grid g, CollideParams<float> P, uint32_t nkb,
+                                                      float rho0, float alfa, float beta) {
+    using S = typename PR::S;
+    constexpr int L = LayoutLIJK;
+    constexpr int STREAM = StreamEsotwist;
+    constexpr int MS = MOMSRC_ESO_POPS;
+    // the gathers read the populations of the next step's collision
+    constexpr int GE = EVEN ? 0 : 1;
+    __shared__ uint32_t s_item;
+    __shared__ int64_t s_pre[258];
+    __shared__ int32_t s_c0[256];
+    const uint32_t x = xcc_id() % kXcds;
+    uint32_t* cx = a.ctr + 64 * x;
+    uint64_t* tx = reinterpret_cast<uint64_t*>(cx + 8);  // phase times (wall clock)
+    uint32_t* rows_done = a.ctr + 64 * kXcds + (size_t)x * (g.xg1 + 2);
+    int jlo, jhi;
+    pipe_range(g.yg1, (int)x, jlo, jhi);
+    const int nj = jhi - jlo + 1, ni = g.xg1;
+    const uint32_t R = (uint32_t)nj * nkb;                // row blocks per i step
+    const uint32_t RG = (R + kPipeRows - 1) / kPipeRows;  // row items per i step
+    const uint32_t nA = (uint32_t)ni * (RG + 1) + 3;
+    const int rw = g.xg1 + 2;
+    const int32_t* rc = a.rowcell;
+    const int64_t cl0 = rc[(size_t)jlo * rw + 1], cl1 = rc[(size_t)jlo * rw + ni + 1];
+    const int64_t ch0 = rc[(size_t)jhi * rw + 1], ch1 = rc[(size_t)jhi * rw + ni + 1];
+    const int64_t chunk = 4 * (int64_t)blockDim.x;
+    const uint32_t nBl = (uint32_t)((cl1 - cl0 + chunk - 1) / chunk), nBh = (uint32_t)((ch1 - ch0 + chunk - 1) / chunk);
+    const uint32_t nB = nBl + nBh;
+    const uint32_t gathers = (uint32_t)ni + nB;
+    const uint32_t total = nA + nB;
+    S* Q = static_cast<S*>(a.Q);
+    float* F = static_cast<float*>(a.F);
+    auto* Nu = static_cast<typename PR::N*>(a.Nu);
+    const int tid = (int)threadIdx.x;
+
+    // gather of row i over the interior planes of the range
+    auto gather_row = [&](int i) {
+        const int ninner = nj - 2;
+        for (int p = tid; p < ninner; p += blockDim.x) {
+            const size_t r = (size_t)(jlo + 1 + p) * rw + i;
+            s_c0[p] = rc[r];
+            s_pre[p + 1] = rc[r + 1] - rc[r];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            s_pre[0] = 0;
+            for (int p = 0; p < ninner; p++) s_pre[p + 1] += s_pre[p];
+        }
+        __syncthreads();
+        const int64_t cells = s_pre[ninner];
+        for (int64_t t = tid; t < cells; t += blockDim.x) {
+            int lo = 0, hi = ninner - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_pre[mid] <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            const int64_t e = s_c0[lo] + (t - s_pre[lo]);
+            gather_cells<PR, L, STREAM, MS, GE>(a.fg, e, e + 1, 0, 1, Q, g, rho0);
+        }
+    };
+    auto wait_rows = [&](int i) {
+        const int im = i == 1 ? ni : i - 1, ip = i == ni ? 1 : i + 1;
+        pipe_wait(rows_done + im, R, rows_done + i, R, rows_done + ip, R, cx + 4);
+    };
+    auto gather_done = [&]() {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        if (tid == 0 &&
+            __hip_atomic_fetch_add(cx + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 == gathers)
+            tx[2] = tx[3] = (uint64_t)wall_clock64();
+    };
+
+    // the next item is taken while the current one runs (each waits only for earlier items)
+    uint32_t ticket = 0;
+    if (tid == 0) ticket = __hip_atomic_fetch_add(cx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) {
+            s_item = ticket;
+            if (ticket == 0) tx[0] = (uint64_t)wall_clock64();
+            if (ticket < total) ticket = __hip_atomic_fetch_add(cx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        const uint32_t u = s_item;
+        if (u >= total) break;
+        if (u < (uint32_t)ni * (RG + 1)) {
+            const int i = (int)(u / (RG + 1)) + 1;
+            const uint32_t r = u % (RG + 1);
+            if (r < RG) {
+                // A: row blocks [r G, r G + G) of i step i: row (i, jlo + b / nkb), k chunk b % nkb
+                const uint32_t b0 = r * kPipeRows, b1 = min(b0 + kPipeRows, R);
+                for (uint32_t b = b0; b < b1; b++) {
+                    const int j = jlo + (int)(b / nkb);
+                    const int k = (int)((b % nkb) * blockDim.x) + 1 + tid;
+                    if (k <= g.zg1)
+                        esotwist_cell<PR, L, MODE_FUSED, EVEN, HINT>(Q, nullptr, F, Nu, a.prev, g, P, rho0, i, j, k);
+                    if (b % nkb == 0) {
+                        // static occupancy words of the row, and of the ghost rows beside it
+                        const int ia = i == 1 ? 0 : i, ib = i == ni ? ni + 1 : i;
+                        const int ja = j == 1 ? 0 : j, jb = j == g.yg1 ? g.yg1 + 1 : j;
+                        const int nw = (int)g.mask_w;
+                        const int cnt = (ib - ia + 1) * (jb - ja + 1) * nw;
+                        for (int t = tid; t < cnt; t += blockDim.x) {
+                            const int w = t % nw, rr = t / nw;
+                            const int jj = ja + rr % (jb - ja + 1), ii = ia + rr / (jb - ja + 1);
+                            const uint64_t wi =
+                                ((uint64_t)ii * (uint64_t)g.yg + (uint64_t)jj) * g.mask_w + (uint64_t)w;
+                            a.next[wi] = a.static_base[wi];
+                        }
+                    }
+                }
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                __syncthreads();
+                if (tid == 0) {
+                    __hip_atomic_fetch_add(rows_done + i, b1 - b0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint32_t done =
+                        __hip_atomic_fetch_add(cx + 1, b1 - b0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (done + (b1 - b0) == (uint32_t)ni * R) {
+                        // every row of the range is in this XCD's L2: write it back for the neighbours
+                        tx[1] = (uint64_t)wall_clock64();
+                        __threadfence();
+                        __hip_atomic_store(cx + 3, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
+            } else if (i >= 4) {
+                wait_rows(i - 2);
+                gather_row(i - 2);
+                gather_done();
+            }
+        } else if (u < nA) {
+            const uint32_t tail = u - (uint32_t)ni * (RG + 1);
+            const int i = tail == 0 ? ni - 1 : tail == 1 ? ni : 1;
+            wait_rows(i);
+            gather_row(i);
+            gather_done();
+        } else if (u < nA + nB) {
+            // B: the edge planes, after the neighbouring ranges are released
+            const uint32_t b = u - nA;
+            const uint32_t xl = (x + kXcds - 1) % kXcds, xh = (x + 1) % kXcds;
+            pipe_wait(a.ctr + 64 * xl + 3, 1, cx + 3, 1, a.ctr + 64 * xh + 3, 1, cx + 4);
+            __threadfence();
+            const int64_t c0 = b < nBl ? cl0 + (int64_t)b * chunk : ch0 + (int64_t)(b - nBl) * chunk;
+            const int64_t c1 = b < nBl ? cl1 : ch1;
+            gather_cells<PR, L, STREAM, MS, GE>(a.fg, c0, c0 + chunk < c1 ? c0 + chunk : c1, tid, blockDim.x, Q, g,
+                                                 rho0);
+            gather_done();
+        }
+    }
+}
+
+__global__ void k_xcc_ids(uint32_t* out) {
+    if (threadIdx.x == 0) out[blockIdx.x] = xcc_id();
 }
 
 // ------------------------------------------------------------------ FP64 sums and macroscopic fields
@@ -1631,6 +2164,106 @@ hipError_t with_cell_source(int layout, int streaming, int momsrc, int even, Fn&
 }
 }  // namespace
 
+hipError_t launch_force_planes(int prec, int layout, int streaming, int momsrc, int even, const ForceGeom& fg,
+                               const void* Q, void* F, const uint64_t* prev, uint64_t* next, const Grid& g,
+                               double alfa, double beta, float rho0, hipStream_t s) {
+    if (fg.n_blocks <= 0) return hipSuccess;
+    hipError_t e = hipErrorInvalidValue;
+    TDLB_WITH_PREC(prec, {
+        using S = typename PR::S;
+        using T = typename PR::T;
+        e = with_cell_source<PR>(layout, streaming, momsrc, even, [&](auto Lc, auto Sc, auto Mc, auto Ec) {
+            hipLaunchKernelGGL((k_force_planes<PR, decltype(Lc)::value, decltype(Sc)::value, decltype(Mc)::value,
+                                               decltype(Ec)::value>),
+                               dim3((uint32_t)fg.n_blocks), dim3(kPlaneBlock), 0, s, fg, (const S*)Q, (T*)F, prev,
+                               next, g, (T)alfa, (T)beta, rho0);
+            return hipGetLastError();
+        });
+    });
+    return e;
+}
+
+bool line_rows_bytes(const Grid& g, size_t ssize) {
+    return ssize == 4 ? line_rows<float>(g) : line_rows<_Float16>(g);
+}
+
+// the pipe's row blocks take at most 256 lanes: 4 waves, so that several
+// workgroups share a CU at the register count of the stream-collide code
+constexpr int kPipeLanes = 256;
+uint32_t row_lanes(const Grid& g) {
+    uint32_t nkb, nb, bk;
+    row_launch(g, nkb, nb, bk, kPipeLanes);
+    return bk;
+}
+
+hipError_t launch_eso_pipe(int prec, int even, int hint, const PipeArgs& a, const Grid& g, const CollideArgs& c,
+                           float rho0, double alfa, double beta, uint32_t blocks, uint32_t lanes, hipStream_t s) {
+    uint32_t nkb, nb, bk;
+    row_launch(g, nkb, nb, bk, kPipeLanes);
+    if (lanes != bk) return hipErrorInvalidValue;
+    auto go = [&](auto pr) -> hipError_t {
+        using PR = decltype(pr);
+        using S = typename PR::S;
+        const auto P = to_params<float, typename PR::D>(c);
+#define TDLB_PIPE(E, H)                                                                                         \
+    hipLaunchKernelGGL((k_eso_pipe<PR, E, H>), dim3(blocks), dim3(bk), 0, s, a, g, P, nkb, rho0, (float)alfa,    \
+                       (float)beta)
+        if (blocks == 0) {
+            // persistent grid: every workgroup resident at once
+            int per_cu = 0, dev = 0;
+            hipDeviceProp_t prop;
+            if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eso_pipe<PR, 1, true>, (int)bk, 0) !=
+                    hipSuccess)
+                return hipErrorInvalidValue;
+            // at least 8 per CU: workgroups beyond the resident ones find the queues drained
+            blocks = (uint32_t)std::max(8, per_cu) * (uint32_t)prop.multiProcessorCount;
+        }
+        if (even) { if (hint) TDLB_PIPE(1, true); else TDLB_PIPE(1, false); }
+        else { if (hint) TDLB_PIPE(0, true); else TDLB_PIPE(0, false); }
+#undef TDLB_PIPE
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess || a.fg.n_blocks <= 0) return e;
+        // the levels read the populations of the next step's collision
+        const S* q = static_cast<const S*>(a.Q);
+        if (even)
+            hipLaunchKernelGGL((k_force_levels<PR, 0>), dim3((uint32_t)a.fg.n_blocks), dim3(kLevelBlock), 0, s, a.fg,
+                               q, static_cast<float*>(a.F), a.prev, a.next, g, (float)alfa, (float)beta, rho0);
+        else
+            hipLaunchKernelGGL((k_force_levels<PR, 1>), dim3((uint32_t)a.fg.n_blocks), dim3(kLevelBlock), 0, s, a.fg,
+                               q, static_cast<float*>(a.F), a.prev, a.next, g, (float)alfa, (float)beta, rho0);
+        return hipGetLastError();
+    };
+    switch (prec) {
+        case 0: return go(PrecF32{});
+        case 2: return go(PrecF16{});
+        case 3: return go(PrecBF16{});
+        case 4: return go(PrecE3M12{});
+        default: return hipErrorInvalidValue;
+    }
+}
+
+int pipe_xcc_check(hipStream_t s) {
+    constexpr int nb = 64 * kXcds;
+    uint32_t* d = nullptr;
+    if (hipMalloc((void**)&d, nb * sizeof(uint32_t)) != hipSuccess) return 0;
+    hipLaunchKernelGGL(k_xcc_ids, dim3(nb), dim3(64), 0, s, d);
+    uint32_t h[nb];
+    const bool ok = hipGetLastError() == hipSuccess &&
+                    hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s) == hipSuccess &&
+                    hipStreamSynchronize(s) == hipSuccess;
+    (void)hipFree(d);
+    if (!ok) return 0;
+    int seen[kXcds] = {0};
+    for (int b = 0; b < nb; b++) {
+        if (h[b] >= (uint32_t)kXcds) return 0;
+        seen[h[b]]++;
+    }
+    for (int x = 0; x < kXcds; x++)
+        if (seen[x] == 0) return 0;
+    return 1;
+}
+
 hipError_t launch_reduce(int prec, int layout, int streaming, int momsrc, int even, const void* Q, const void* F,
                          const uint64_t* fmask, const Grid& g, const ReduceArgs& ra, float rho0, double* partial,
                          double* out, hipStream_t s) {
@@ -1796,7 +2429,7 @@ __device__ __forceinline__ int find_slot(const int64_t* __restrict__ off, int n,
 __device__ __forceinline__ bool batch_cell(const Grid& g, int tile, uint32_t b, int& i, int& j, int& k) {
     if (tile == TILE_ROW64) {
         const uint32_t segs = (uint32_t)(g.zg1 + 63) >> 6;
-        const uint32_t w = b * (kBlock / 64) + (threadIdx.x >> 6);
+        const uint32_t w = b * (blockDim.x >> 6) + (threadIdx.x >> 6);
         const uint32_t rows = (uint32_t)g.xg1 * (uint32_t)g.yg1;
         if (w >= rows * segs) return false;
         const uint32_t row = w / segs, seg = w - row * segs;
@@ -1806,10 +2439,20 @@ __device__ __forceinline__ bool batch_cell(const Grid& g, int tile, uint32_t b, 
         k = (int)(seg * 64 + (threadIdx.x & 63)) + 1;
         return k <= g.zg1;
     }
+    if (tile == TILE_FLAT) {
+        const uint32_t plane = (uint32_t)g.yg1 * (uint32_t)g.zg1;
+        const uint64_t t = (uint64_t)b * blockDim.x + threadIdx.x;
+        if (t >= (uint64_t)g.xg1 * plane) return false;
+        const uint32_t ip = (uint32_t)(t / plane), r = (uint32_t)(t - (uint64_t)ip * plane);
+        i = (int)ip + 1;
+        j = (int)(r / (uint32_t)g.zg1) + 1;
+        k = (int)(r - (uint32_t)(j - 1) * (uint32_t)g.zg1) + 1;
+        return true;
+    }
     const uint32_t plane = (uint32_t)g.yg * (uint32_t)g.zg;
-    const uint32_t bpp = (plane + kBlock - 1) / kBlock;
+    const uint32_t bpp = (plane + blockDim.x - 1) / blockDim.x;
     const uint32_t ip = b / bpp;
-    const uint32_t local = (b - ip * bpp) * kBlock + threadIdx.x;
+    const uint32_t local = (b - ip * bpp) * blockDim.x + threadIdx.x;
     if (ip >= (uint32_t)g.xg1 || local >= plane) return false;
     i = (int)ip + 1;
     j = (int)(local / (uint32_t)g.zg);
@@ -1818,7 +2461,7 @@ __device__ __forceinline__ bool batch_cell(const Grid& g, int tile, uint32_t b, 
 }
 
 template <typename PR, int L, int STREAM>
-__global__ __launch_bounds__(kBlock) void k_batch_fused(const MemberDev* __restrict__ mem,
+__global__ __launch_bounds__(kRowBlock) void k_batch_fused(const MemberDev* __restrict__ mem,
                                                         const int64_t* __restrict__ off, int n) {
     static_assert(std::is_same<typename PR::T, float>::value, "batches compute in float");
     using S = typename PR::S;
@@ -1849,7 +2492,7 @@ __global__ void k_batch_blades(const MemberDev* __restrict__ mem, const int64_t*
     const int m = find_slot(off, n, t);
     const MemberDev& md = mem[m];
     if (!md.active) return;
-    blade_gen_point<float>(md.blade, md.steps[md.done], md.pts, md.n_static, t - off[m]);
+    blade_gen_point<float>(md.fg.blade, md.steps[md.done], md.fg.pts, md.fg.n_static, t - off[m]);
 }
 
 // the new occupancy mask starts from the static stencil bits
@@ -1863,92 +2506,33 @@ __global__ void k_batch_mask_init(const MemberDev* __restrict__ mem, const int64
     md.fmask[1 - md.fm][t - off[m]] = md.static_base[t - off[m]];
 }
 
-__global__ void k_batch_prepare(const MemberDev* __restrict__ mem, const int64_t* __restrict__ off, int n,
-                                int64_t total) {
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= total || t >= off[n]) return;
-    const int m = find_slot(off, n, t);
-    const MemberDev& md = mem[m];
-    if (!md.active) return;
-    prepare_point(md.pts, md.n_static + (t - off[m]), md.F, md.fmask[md.fm], md.fmask[1 - md.fm], md.g);
-}
-
-// density and momentum of a cell from the populations the fused kernel just wrote
+// the forcing planes of every member: one workgroup per (member, plane) task,
+// density and momentum from the populations the fused kernel just wrote
 template <typename PR, int L, int STREAM>
-__device__ __forceinline__ void batch_rho_j(const MemberDev& md, int i, int j, int k, Mom4<float>& m) {
+__global__ __launch_bounds__(kPlaneBlock) void k_batch_force_planes(const MemberDev* __restrict__ mem,
+                                                                    const PlaneTask* __restrict__ tasks) {
+    const PlaneTask tk = tasks[blockIdx.x];
+    const MemberDev& md = mem[tk.member];
+    if (!md.active) return;
     using S = typename PR::S;
+    const PlaneBlock pb = md.fg.blocks[tk.block];
+    if (pb.plane < 0) return;
+    const PlaneDesc& pd = md.fg.planes[pb.plane];
+    const uint64_t* prev = md.fmask[md.fm];
+    uint64_t* next = md.fmask[1 - md.fm];
     if constexpr (STREAM == StreamEsotwist) {
         const S* Q = static_cast<const S*>(md.Q[0]);
-        if (md.even) cell_rho_j<PR, L, STREAM, MOMSRC_ESO_POPS, 0>(Q, md.g, i, j, k, m.r, m.x, m.y, m.z, md.rho0);
-        else cell_rho_j<PR, L, STREAM, MOMSRC_ESO_POPS, 1>(Q, md.g, i, j, k, m.r, m.x, m.y, m.z, md.rho0);
+        if (md.even)
+            force_plane<PR, L, STREAM, MOMSRC_ESO_POPS, 0>(md.fg, pd, pb.rank, Q, md.F, prev, next, md.g, md.alfa, md.beta,
+                                                            md.rho0);
+        else
+            force_plane<PR, L, STREAM, MOMSRC_ESO_POPS, 1>(md.fg, pd, pb.rank, Q, md.F, prev, next, md.g, md.alfa, md.beta,
+                                                            md.rho0);
     } else {
         const S* Q = static_cast<const S*>(md.Q[1 - md.cur]);
-        cell_rho_j<PR, L, STREAM, MOMSRC_PULL_POPS, 1>(Q, md.g, i, j, k, m.r, m.x, m.y, m.z, md.rho0);
+        force_plane<PR, L, STREAM, MOMSRC_PULL_POPS, 1>(md.fg, pd, pb.rank, Q, md.F, prev, next, md.g, md.alfa, md.beta,
+                                                         md.rho0);
     }
-}
-
-template <typename PR, int L, int STREAM>
-__global__ __launch_bounds__(kBlock) void k_batch_moments_static(const MemberDev* __restrict__ mem,
-                                                                 const int64_t* __restrict__ off, int n,
-                                                                 int64_t total) {
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= total || t >= off[n]) return;
-    const int m = find_slot(off, n, t);
-    const MemberDev& md = mem[m];
-    if (!md.active) return;
-    const int64_t e = t - off[m];
-    const uint32_t c = md.scells[e];
-    const int i = (int)(c / md.g.yz);
-    const uint32_t r = c - (uint32_t)i * md.g.yz;
-    const int j = (int)(r / (uint32_t)md.g.zg);
-    const int k = (int)(r - (uint32_t)j * (uint32_t)md.g.zg);
-    Mom4<float> mm;
-    batch_rho_j<PR, L, STREAM>(md, i, j, k, mm);
-    static_cast<Mom4<float>*>(md.scache)[e] = mm;
-}
-
-// off counts blade points; thread t handles stencil cell t % 9 of point t / 9
-template <typename PR, int L, int STREAM>
-__global__ __launch_bounds__(kBlock) void k_batch_moments_blades(const MemberDev* __restrict__ mem,
-                                                                 const int64_t* __restrict__ off, int n,
-                                                                 int64_t total9) {
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= total9 || t >= 9 * off[n]) return;
-    const int m = find_slot(off, n, t / 9);
-    const MemberDev& md = mem[m];
-    if (!md.active) return;
-    const int64_t e = t - 9 * off[m];
-    const int64_t q = md.n_static + e / 9;
-    const int s = (int)(e % 9);
-    const int i2 = wrap_ik(md.pts.i[q] + (s % 3) - 1, md.g.xg1), k2 = wrap_ik(md.pts.k[q] + (s / 3) - 1, md.g.zg1);
-    Mom4<float> mm;
-    batch_rho_j<PR, L, STREAM>(md, i2, md.pts.j[q], k2, mm);
-    static_cast<Mom4<float>*>(md.rcache)[e] = mm;
-}
-
-// one forcing level over the segments [sb, se) (static lists INDEXED, blades direct)
-template <bool INDEXED>
-__global__ __launch_bounds__(kBlock) void k_batch_force_level(const MemberDev* __restrict__ mem,
-                                                              const LevelSeg* __restrict__ segs,
-                                                              const int64_t* __restrict__ cum, int64_t sb, int ns,
-                                                              int64_t total) {
-    int sl, base;
-    const int64_t t = group_point(sl, base);
-    bool valid = t >= 0 && t < total;
-    // every lane of the wavefront reaches the group shuffles; lanes without a point pass valid = false
-    const int64_t s = valid ? sb + find_slot(cum + sb, ns, t) : sb;
-    const LevelSeg sg = segs[s];
-    valid = valid && t - cum[s] < sg.count;
-    const MemberDev& md = mem[sg.member];
-    valid = valid && md.active;
-    const int64_t q = valid ? sg.begin + (t - cum[s]) : 0;
-    if (INDEXED)
-        force_point9<float, double, true>(md.pts, q, valid, sl, base, md.sidx, 0,
-                                          static_cast<const Mom4<float>*>(md.scache), md.F, md.g, md.alfa, md.beta);
-    else
-        force_point9<float, double, false>(md.pts, q, valid, sl, base, nullptr, md.n_static,
-                                           static_cast<const Mom4<float>*>(md.rcache), md.F, md.g, md.alfa,
-                                           md.beta);
 }
 
 template <int STREAM>
@@ -1969,7 +2553,7 @@ inline uint32_t nblk(int64_t n, int bs) { return (uint32_t)((n + bs - 1) / bs); 
 template <typename PR, int L, int STREAM>
 hipError_t batch_fused_t(const BatchPlan& p, hipStream_t s) {
     if (p.tot_coll <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_batch_fused<PR, L, STREAM>), dim3((uint32_t)p.tot_coll), dim3(kBlock), 0, s, p.mem,
+    hipLaunchKernelGGL((k_batch_fused<PR, L, STREAM>), dim3((uint32_t)p.tot_coll), dim3(p.bs), 0, s, p.mem,
                        p.off_coll, p.n_members);
     return hipGetLastError();
 }
@@ -1977,6 +2561,7 @@ hipError_t batch_fused_t(const BatchPlan& p, hipStream_t s) {
 template <typename PR, int L, int STREAM>
 hipError_t batch_step_t(const BatchPlan& p, hipStream_t s) {
     const int n = p.n_members;
+    if (p.bs != 256 && p.bs != 512) return hipErrorInvalidValue;
     if (p.tot_rot > 0)
         hipLaunchKernelGGL(k_batch_blades, dim3(nblk(p.tot_rot, kBlock)), dim3(kBlock), 0, s, p.mem, p.off_rot, n,
                            p.tot_rot);
@@ -1985,28 +2570,9 @@ hipError_t batch_step_t(const BatchPlan& p, hipStream_t s) {
     if (p.tot_words > 0)
         hipLaunchKernelGGL(k_batch_mask_init, dim3(nblk(p.tot_words, kBlock)), dim3(kBlock), 0, s, p.mem,
                            p.off_words, n, p.tot_words);
-    if (p.tot_rot > 0)
-        hipLaunchKernelGGL(k_batch_prepare, dim3(nblk(p.tot_rot, kBlock)), dim3(kBlock), 0, s, p.mem, p.off_rot, n,
-                           p.tot_rot);
-    if (p.tot_scells > 0)
-        hipLaunchKernelGGL((k_batch_moments_static<PR, L, STREAM>), dim3(nblk(p.tot_scells, kBlock)), dim3(kBlock),
-                           0, s, p.mem, p.off_scells, n, p.tot_scells);
-    if (p.tot_rot > 0)
-        hipLaunchKernelGGL((k_batch_moments_blades<PR, L, STREAM>), dim3(nblk(9 * p.tot_rot, kBlock)), dim3(kBlock),
-                           0, s, p.mem, p.off_rot, n, 9 * p.tot_rot);
-    constexpr int bs = 64;  // levels are small: more blocks spread over more CUs
-    for (int l = 0; l < p.n_slevels; l++) {
-        const int64_t sb = p.lv_static[l], ns = p.lv_static[l + 1] - sb, tot = p.lv_threads[l];
-        if (tot > 0)
-            hipLaunchKernelGGL(k_batch_force_level<true>, dim3(group_blocks(tot, bs)), dim3(bs), 0, s, p.mem,
-                               p.segs, p.seg_cum, sb, (int)ns, tot);
-    }
-    for (int l = 0; l < p.n_rlevels; l++) {
-        const int64_t sb = p.lv_blade[l], ns = p.lv_blade[l + 1] - sb, tot = p.lv_threads[p.n_slevels + l];
-        if (tot > 0)
-            hipLaunchKernelGGL(k_batch_force_level<false>, dim3(group_blocks(tot, bs)), dim3(bs), 0, s, p.mem,
-                               p.segs, p.seg_cum, sb, (int)ns, tot);
-    }
+    if (p.n_tasks > 0)
+        hipLaunchKernelGGL((k_batch_force_planes<PR, L, STREAM>), dim3((uint32_t)p.n_tasks), dim3(kPlaneBlock), 0, s,
+                           p.mem, p.tasks);
     hipLaunchKernelGGL(k_batch_advance<STREAM>, dim3(nblk(n, 64)), dim3(64), 0, s, p.mem_rw, n);
     return hipGetLastError();
 }
@@ -2031,19 +2597,23 @@ hipError_t with_batch_prec(int prec, int layout, int streaming, Fn&& fn) {
 }
 }  // namespace
 
-int64_t batch_coll_blocks(const Grid& g, int tile) {
+int64_t batch_coll_blocks(const Grid& g, int tile, int bs) {
     if (tile == TILE_ROW64) {
         const int64_t segs = (g.zg1 + 63) / 64;
         const int64_t waves = (int64_t)g.xg1 * g.yg1 * segs;
-        return (waves + kBlock / 64 - 1) / (kBlock / 64);
+        return (waves + bs / 64 - 1) / (bs / 64);
     }
+    if (tile == TILE_FLAT) return ((int64_t)g.xg1 * g.yg1 * g.zg1 + bs - 1) / bs;
     const int64_t plane = (int64_t)g.yg * g.zg;
-    return (int64_t)g.xg1 * ((plane + kBlock - 1) / kBlock);
+    return (int64_t)g.xg1 * ((plane + bs - 1) / bs);
 }
 
-hipError_t launch_batch_step(int prec, int layout, int streaming, const BatchPlan& plan, hipStream_t s) {
+hipError_t launch_batch_step(int prec, int layout, int streaming, const BatchPlan& plan, int steps, hipStream_t s) {
     return with_batch_prec(prec, layout, streaming, [&](auto pr, auto lc, auto sc) {
-        return batch_step_t<decltype(pr), decltype(lc)::value, decltype(sc)::value>(plan, s);
+        hipError_t e = hipSuccess;
+        for (int k = 0; k < steps && e == hipSuccess; k++)
+            e = batch_step_t<decltype(pr), decltype(lc)::value, decltype(sc)::value>(plan, s);
+        return e;
     });
 }
 

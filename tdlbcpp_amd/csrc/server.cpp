@@ -120,6 +120,9 @@ void prepare(tdlb_server* s, Job& j) {
     p.streaming = s->streaming;
     p.precision = s->prec;
     if (rc == TDLB_OK) rc = tdlb_cu_create(&p, &f, &j.cu);
+    // the tile timings of a new grid shape run here, on the tank's own arrays
+    // before they are initialised: no scratch lattice, nothing on the tick thread
+    if (rc == TDLB_OK) rc = tdlb_batch_tune(s->batch, j.cu);
     if (rc == TDLB_OK) rc = tdlb_cu_initialise(j.cu, 1.0);
     if (rc == TDLB_OK) rc = tdlb_rushton_defaults(T, T, j.req.tip_speed, j.req.startup_steps, &j.rp);
     if (rc == TDLB_OK) rc = tdlb_cu_bind_rushton(j.cu, &j.rp, 1, 0.0, f.alpha, f.beta);

@@ -55,7 +55,8 @@ def test_batch_members_equal_solo_runs(cfg):
     assert b.run(10) == 10
     assert b.run(30) == 15
     info = b.info()
-    assert info["members"] == 4 and all(t in (0, 1) for t in info["tiles"])
+    assert info["members"] == 4 and all(t in (0, 1, 2) for t in info["tiles"])
+    assert info["block_size"] in (256, 512)
     for cu, n in zip(solo, budgets):
         cu.run(n)
     for a, s, n in zip(mem, solo, budgets):
@@ -121,3 +122,30 @@ def test_cache_hinted_solo_run_equals_unhinted_batch_member(prec):
     for x, y in zip(fields(mem), fields(solo)):
         np.testing.assert_array_equal(x, y)
     b.close()
+
+
+def test_batch_of_64_uneven_fp16_tanks_equals_solo_runs():
+    """The batched-tank workload at its shape: 64 FP16 Esotwist/LIJK tanks with
+    edges drawn from {48, 64, 96, 128, 160}, tip speeds and Reynolds numbers as
+    the bench draws them (numpy PCG64, seed 1), graph replay and autotuning on,
+    10 steps: every member equals its solo run bit for bit."""
+    import bench
+    specs = bench.batch_tanks(64, steps=10, seed=1)
+    case(f"batch of {len(specs)} fp16 tanks, edges {sorted(set(s['edge'] for s in specs))}")
+    prec, st, ly = td.PREC_F16, td.STREAM_ESOTWIST, td.LAYOUT_LIJK
+    mem = [make_tank(s["edge"], prec, st, ly, tip=s["tip"], re=s["re"], startup=100) for s in specs]
+    b = td.Batch(prec, ly, st, use_graph=True, autotune=True)
+    for cu in mem:
+        b.add(cu, 10)
+    assert b.run(10) == 10
+    info = b.info()
+    print(f"[gpu case] block size {info['block_size']}, tiles {info['tiles']}", flush=True)
+    b.close()
+    for cu, s in zip(mem, specs):
+        solo = make_tank(s["edge"], prec, st, ly, tip=s["tip"], re=s["re"], startup=100)
+        solo.run(10)
+        assert (cu.step, cu.angle) == (solo.step, solo.angle)
+        for x, y in zip(fields(cu), fields(solo)):
+            np.testing.assert_array_equal(x, y)
+        solo.close()
+        cu.close()

@@ -169,6 +169,82 @@ def test_gpu_main_loop_20_steps_matches_oracle(layout, stream):
     assert exact
 
 
+@pytest.mark.parametrize("T,steps,team_cost", [(64, 20, None), (64, 10, "8"), (128, 6, None)],
+                         ids=["64", "64-teams", "128"])
+def test_gpu_one_launch_step_matches_oracle(T, steps, team_cost, monkeypatch):
+    """Esotwist/LIJK tanks of 64^3 and more can take the one-launch step (the fused
+    stream-collide and the IB forcing gather in one persistent kernel, XCD by XCD,
+    then the forcing levels): bit-identical to the oracle, also with every plane's
+    levels run by a team of workgroups (a low TDLB_FORCE_TEAM_COST)."""
+    case(f"one-launch step T={T} {steps} steps Esotwist/LIJK fp32 team cost {team_cost}")
+    monkeypatch.setenv("TDLB_PIPE", "1")
+    if team_cost:
+        monkeypatch.setenv("TDLB_FORCE_TEAM_COST", team_cost)
+    oflow, gflow = flow_for_tank(T)
+    o = O.OracleCU(T, dtype=O.F32, layout=O.LIJK, streaming=O.ESOTWIST, use_les=True, **oflow)
+    g = td.ComputeUnit(T, layout=td.LAYOUT_LIJK, streaming=td.STREAM_ESOTWIST, precision=td.PREC_F32, **gflow)
+    rp = td.rushton_defaults(T)
+    o.initialise(1.0)
+    g.initialise(1.0)
+    ref = ReferenceLoop(o, rp)
+    g.bind_rushton(rp, step=1, angle=0.0)
+    assert g.one_launch
+    ref.run(steps)
+    g.run(steps)
+    g.synchronize()  # reports a wait of the step kernel that timed out
+    np.testing.assert_array_equal(g.get(td.FIELD_F), o.get(O.FIELD_F))
+    np.testing.assert_array_equal(g.get(td.FIELD_NU), o.get(O.FIELD_NU))
+    np.testing.assert_array_equal(g.get(td.FIELD_Q), o.get(O.FIELD_Q))
+
+
+@pytest.mark.parametrize("prec", [td.PREC_F16, td.PREC_BF16, td.PREC_E3M12], ids=["fp16", "bf16", "e3m12"])
+def test_gpu_one_launch_step_equals_separate_launches(prec, monkeypatch):
+    """16-bit storage (no oracle): the one-launch step leaves the state of the
+    separate stream-collide and forcing launches, bit for bit."""
+    T, steps = 96, 12
+    case(f"one-launch vs separate launches T={T} {steps} steps prec={prec}")
+    _, gflow = flow_for_tank(T)
+    rp = td.rushton_defaults(T)
+    out = []
+    for pipe in ("1", "0"):
+        monkeypatch.setenv("TDLB_PIPE", pipe)
+        g = td.ComputeUnit(T, layout=td.LAYOUT_LIJK, streaming=td.STREAM_ESOTWIST, precision=prec, **gflow)
+        g.initialise(1.0)
+        g.bind_rushton(rp, step=1, angle=0.0)
+        assert g.one_launch == (pipe == "1")
+        g.run(steps)
+        g.synchronize()
+        out.append([g.get(f) for f in (td.FIELD_Q, td.FIELD_F, td.FIELD_NU)])
+        g.close()
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("stream", STREAMS, ids=["Simple", "Esotwist"])
+def test_gpu_forcing_plane_teams_match_oracle(stream, monkeypatch):
+    """Planes forced by teams of workgroups meeting at a barrier in global memory
+    (the impeller disk's plane of a large tank) give the sequential result: a low
+    TDLB_FORCE_TEAM_COST gives every plane of a 48^3 tank a team of about four
+    workgroups (about 190 in all, every one resident at once)."""
+    T = 48
+    case(f"forcing teams T={T} 12 steps {stream}")
+    monkeypatch.setenv("TDLB_FORCE_TEAM_COST", "8")
+    oflow, gflow = flow_for_tank(T)
+    o = O.OracleCU(T, dtype=O.F32, layout=O.LIJK, streaming=stream[1], use_les=True, **oflow)
+    g = td.ComputeUnit(T, layout=td.LAYOUT_LIJK, streaming=stream[0], precision=td.PREC_F32, **gflow)
+    rp = td.rushton_defaults(T)
+    o.initialise(1.0)
+    g.initialise(1.0)
+    ref = ReferenceLoop(o, rp)
+    g.bind_rushton(rp, step=1, angle=0.0)
+    monkeypatch.delenv("TDLB_FORCE_TEAM_COST")
+    ref.run(12)
+    g.run(12)
+    g.synchronize()  # reports a team barrier that timed out
+    np.testing.assert_array_equal(g.get(td.FIELD_F), o.get(O.FIELD_F))
+    np.testing.assert_array_equal(g.get(td.FIELD_Q), o.get(O.FIELD_Q))
+
+
 @pytest.mark.parametrize("stream", STREAMS, ids=["Simple", "Esotwist"])
 def test_gpu_eager_calls_equal_fused_run(stream):
     """The five reference calls per step and the fused fast path leave identical state."""

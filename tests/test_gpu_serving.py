@@ -86,3 +86,39 @@ def test_admission_respects_the_hbm_budget():
     with pytest.raises(td.TdlbError):
         s.submit(160, 0.1, 5000.0, 10)  # larger than the whole budget
     s.close()
+
+
+def test_served_burst_of_uneven_tanks_up_to_320():
+    """The served workload at its shape: a burst of FP16 Esotwist/LIJK requests
+    with edges log-uniform in [64, 320] as the bench draws them, plus one 320^3
+    tank, admitted by footprint against the HBM budget and run with continuous
+    batching; sampled results (the 320^3 tank among them) equal solo runs bit
+    for bit, and every request completes."""
+    import bench
+    prec, st, ly = td.PREC_F16, td.STREAM_ESOTWIST, td.LAYOUT_LIJK
+    draws = bench.serve_requests(4.0, rate=4.0, seed=1)[:12]
+    reqs = [(r["edge"], r["tip"], r["re"], 8 + 3 * k) for k, r in enumerate(draws)] + [(320, 0.1, 1.0e4, 12)]
+    case(f"served burst of {len(reqs)} tanks, edges {[r[0] for r in reqs]}")
+    s = td.Server(prec, ly, st, tick_steps=5, keep_handles=True)
+    tickets = [s.submit(*r) for r in reqs[:6]]
+    results = []
+    for _ in range(2):
+        s.tick()
+        results += s.poll()
+    tickets += [s.submit(*r) for r in reqs[6:]]  # arrive while the first ones run
+    results += drain(s)
+    assert sorted(r["ticket"] for r in results) == sorted(tickets)
+    assert all(r["error"] == 0 for r in results)
+    by_ticket = {r["ticket"]: r for r in results}
+    sample = [0, len(reqs) // 2, len(reqs) - 1]
+    for k in range(len(reqs)):
+        a = s.take(tickets[k], reqs[k][0])
+        if k in sample:
+            T, tip, re, steps = reqs[k]
+            assert by_ticket[tickets[k]]["steps"] == steps
+            b = solo(T, tip, re, steps, prec, st, ly)
+            for f in (td.FIELD_Q, td.FIELD_F, td.FIELD_NU, td.FIELD_O):
+                np.testing.assert_array_equal(a.get(f), b.get(f))
+            b.close()
+        a.close()
+    s.close()
