@@ -118,6 +118,7 @@ def lib():
             "tdlb_cu_profile": ([vp, i32], i32),
             "tdlb_cu_profile_read": ([vp, P(d), P(i64), P(d), P(i64)], i32),
             "tdlb_cu_reductions": ([vp, P(d)], i32),
+            "tdlb_cu_reductions_noalloc": ([vp, vp], i32),
             "tdlb_cu_macroscopic": ([vp, vp, ctypes.c_size_t], i32),
             "tdlb_cu_double_resolution": ([vp], i32),
             "tdlb_cu_output_exclude": ([vp, vp, ctypes.c_size_t], i32),

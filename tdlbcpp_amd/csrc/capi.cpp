@@ -59,6 +59,47 @@ int set_device(tdlb_cu* cu) {
     return TDLB_OK;
 }
 
+// ---- scratch of tdlb_cu_reductions_noalloc: the partial sums of the fixed
+// reduction grid (kReduceBlocks rows of kReduceVals) and the result row,
+// 65,600 bytes.  Every GPU the library runs on holds its own copy until the
+// process ends; the host never passes the array's name to a kernel (in host
+// code it is a stand-in, not a device address) but the address
+// hipGetSymbolAddress returns on each device.
+constexpr size_t kScratchDoubles = (size_t)(kReduceBlocks + 1) * kReduceVals;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kScratchDevices] = {};
+
+// the address of scratch on `device`, resolved once per device with that
+// device current (tdlb_cu_create calls this)
+int resolve_scratch(int device) {
+    if (device < 0 || device >= kScratchDevices)
+        return fail(TDLB_ERR_UNSUPPORTED, "device id beyond " + std::to_string(kScratchDevices - 1));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return TDLB_OK;
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(TDLB_ERR_HIP, "hipGetSymbolAddress returned no address for the reduction scratch");
+    g_scratch[device] = (double*)p;
+    return TDLB_OK;
+}
+
+// torque region of the reductions: cells within blade_outer_radius + 2 of the
+// bound tank's axis (none when no geometry is bound)
+ReduceArgs reduce_args(const tdlb_cu* cu) {
+    ReduceArgs ra{};
+    ra.ghost = (int)cu->prm.ghost;
+    ra.r2 = -1.0;
+    if (cu->bound) {
+        const double r = cu->rp.blade_outer_radius + 2.0;
+        ra.cx = cu->rp.center_x;
+        ra.cz = cu->rp.center_z;
+        ra.r2 = r * r;
+    }
+    return ra;
+}
+
 void release(tdlb_cu* cu) {
     void* ptrs[] = {cu->Q[0], cu->Q[1], cu->M, cu->F, cu->Nu, cu->O, cu->fmask[0], cu->fmask[1],
                     cu->gmask, cu->static_bits, cu->static_base, cu->red, cu->rowcell, cu->pctr};
@@ -739,6 +780,10 @@ int tdlb_cu_create(const tdlb_cu_params* p, const tdlb_flow* f, tdlb_cu_t* out) 
         return fail(code, m);
     };
     if (hipSetDevice(cu->device) != hipSuccess) return bad(TDLB_ERR_HIP, "hipSetDevice failed");
+    if (resolve_scratch(cu->device) != TDLB_OK) {
+        const std::string m = g_err;
+        return bad(TDLB_ERR_HIP, m);
+    }
     if (hipStreamCreateWithFlags(&cu->stream, hipStreamNonBlocking) != hipSuccess)
         return bad(TDLB_ERR_HIP, "hipStreamCreate failed");
     const int nq = cu->simple() ? 2 : 1;
@@ -1403,21 +1448,30 @@ int tdlb_cu_reductions(tdlb_cu_t cu, double* out) {
     CHK(set_device(cu));
     if (!cu->red && hipMalloc((void**)&cu->red, sizeof(double) * (kReduceBlocks + 1) * kReduceVals) != hipSuccess)
         return fail(TDLB_ERR_NOMEM, "reduction buffer");
-    ReduceArgs ra{};
-    ra.ghost = (int)cu->prm.ghost;
-    ra.r2 = -1.0;
-    if (cu->bound) {
-        const double r = cu->rp.blade_outer_radius + 2.0;
-        ra.cx = cu->rp.center_x;
-        ra.cz = cu->rp.center_z;
-        ra.r2 = r * r;
-    }
+    const ReduceArgs ra = reduce_args(cu);
     const CellSource cs = cell_source(cu);
     double* res = cu->red + (size_t)kReduceBlocks * kReduceVals;
     HIPCHK(launch_reduce(cs.prec, cu->dlay(), cu->prm.streaming, cs.momsrc, cs.even, cs.Q, cu->F,
                          cu->fmask[cu->fm], cu->g, ra, cu->rho0, cu->red, res, cu->stream));
     HIPCHK(hipMemcpyAsync(out, res, sizeof(double) * kReduceVals, hipMemcpyDeviceToHost, cu->stream));
     HIPCHK(hipStreamSynchronize(cu->stream));
+    return TDLB_OK;
+}
+
+int tdlb_cu_reductions_noalloc(tdlb_cu_t cu, double* out_device) {
+    if (!cu || !out_device) return fail(TDLB_ERR_INVALID, "null argument");
+    static_assert(kScratchDoubles >= (size_t)(kReduceBlocks + 1) * kReduceVals,
+                  "the reduction scratch holds every partial row of the fixed grid and the result");
+    if (cu->device < 0 || cu->device >= kScratchDevices || !g_scratch[cu->device])
+        return fail(TDLB_ERR_STATE, "reduction scratch not resolved on this device");
+    CHK(set_device(cu));
+    const ReduceArgs ra = reduce_args(cu);
+    // every partial row and the result are written before they are read, so
+    // the shared scratch needs no clearing between calls
+    double* partial = g_scratch[cu->device];
+    const CellSource cs = cell_source(cu);
+    HIPCHK(launch_reduce(cs.prec, cu->dlay(), cu->prm.streaming, cs.momsrc, cs.even, cs.Q, cu->F,
+                         cu->fmask[cu->fm], cu->g, ra, cu->rho0, partial, out_device, cu->stream));
     return TDLB_OK;
 }
 
