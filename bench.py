@@ -14,10 +14,16 @@ value = interior cells x steps / seconds / 1e6 (MLUPS), timed with HIP events
 on the ComputeUnit's stream around exactly --steps steps after --warmup steps.
 The roofline entry uses the stream-collide kernel's own event-timed average
 duration and its algorithmic bytes (2 x 18 x 4 B populations + 2 x 4 B Nu per
-cell).  cpu_baseline times the CPU oracle (a restatement of the reference's
-single-threaded CPU path) on one host core on a bounded sample, the same tank
-at 96^3 for ~15 s, and reports beside it, labelled, the oracle with collision
-and moments on OpenMP threads, and the host CPU.
+cell); roofline.copy_gbs is the speed of light of that access pattern, the
+faster of two schedules of a probe that reads and writes back 19 planes of
+grid^3 values in place (tdlb_probe_planes).  accuracy: the 16-bit storage
+formats against the fp64 build of the same kernels on the 96^3 tank over 5000
+steps (velocity L2 at the last step and of the mean field).  serving: a short
+Poisson stream of tank requests below capacity, drained, p50/p99 over every
+request.  cpu_baseline times the CPU oracle (a restatement of the reference's
+single-threaded CPU path) on one host core: the headline tank for 3 steps
+(extrapolated), the 60^3 tank for its full 20 steps, and, labelled apart, the
+60^3 run with collision and moments on OpenMP threads; and the host CPU.
 """
 import argparse
 import ctypes
@@ -119,45 +125,47 @@ def host_cpu():
     return {"cpu": model, "nproc": os.cpu_count(), "cpus_available": avail}
 
 
-def cpu_baseline(sample_grid=96, max_seconds=20.0, omp_seconds=8.0, omp_threads=16):
-    """The oracle (CPU restatement of the reference path) on the same tank at
-    sample_grid^3, Simple streaming, IJKL, FP32, LES: the configuration main.cpp
-    runs on the CPU.  The baseline is one host core, as the reference is
-    single-threaded; a second leg runs the oracle's collision and moments on
-    OpenMP threads (push, bounce-back and forcing stay serial), labelled apart."""
+def cpu_baseline(grid=512, steps=3, omp_threads=16):
+    """The oracle (CPU restatement of the reference path, one host core, as the
+    reference's CPU build is single-threaded) on the reference's CPU
+    configuration: Simple streaming, IJKL, FP32, LES cs0 = 0.1, Rushton tank at
+    rest.  Legs: the headline tank at grid^3 for `steps` main-loop steps (the
+    value: a sample of the 1000-step run, extrapolated), the 60^3 tank at
+    Re 7300 for its full 20 steps, and that 60^3 run again with the oracle's
+    collision and moments on OpenMP threads (push, bounce-back and forcing
+    serial), labelled apart."""
     import tdlbcpp_amd as td
     from oracle import oracle as O
     from tests.helpers import ReferenceLoop, flow_for_tank
-    oflow, _ = flow_for_tank(sample_grid, re=7300.0)
-    rp = td.rushton_defaults(sample_grid)
 
-    def timed(threads, seconds):
+    def timed(G, n, threads, re):
         O.set_threads(threads)
-        o = O.OracleCU(sample_grid, dtype=O.F32, layout=O.IJKL, streaming=O.SIMPLE, use_les=True, **oflow)
+        oflow, _ = flow_for_tank(G, re=re)
+        o = O.OracleCU(G, dtype=O.F32, layout=O.IJKL, streaming=O.SIMPLE, use_les=True, **oflow)
         o.initialise(1.0)
-        loop = ReferenceLoop(o, rp)
-        loop.run(1)  # warm caches
-        steps, t0 = 0, time.time()
-        while True:
-            loop.run(1)
-            steps += 1
-            el = time.time() - t0
-            if el > seconds or steps >= 500:
-                break
-        return sample_grid ** 3 * steps / el / 1e6, steps, el
+        loop = ReferenceLoop(o, td.rushton_defaults(G))  # binds the geometry: its first forcing pass
+        t0 = time.time()
+        loop.run(n)
+        el = time.time() - t0
+        del loop, o
+        return G ** 3 * n / el / 1e6, el
 
     host = host_cpu()
     try:
-        mlups, steps, el = timed(1, max_seconds)
+        mlups, el = timed(grid, steps, 1, 1.0e5)
         out = dict(value=round(mlups, 3), unit="MLUPS", cores=1, kind="port",
-                   sample=f"oracle/liboracle.so: Rushton tank {sample_grid}^3, Simple/IJKL, FP32, LES, "
-                          f"{steps} main-loop steps in {el:.1f} s on one host core", host=host)
+                   sample=f"oracle/liboracle.so: Rushton tank {grid}^3 at Re 1e5, Simple/IJKL, FP32, LES, {steps} "
+                          f"main-loop steps in {el:.1f} s on one host core (a sample of the 1000-step run: "
+                          f"1000 steps extrapolate to {1000 * el / steps / 3600:.1f} h)", host=host)
+        c1, el1 = timed(60, 20, 1, 7300.0)
+        out["config1"] = dict(value=round(c1, 3), unit="MLUPS", cores=1,
+                              sample=f"60^3 tank at Re 7300, Simple/IJKL, FP32, LES: the full 20 steps in {el1:.2f} s")
         threads = min(omp_threads, host["cpus_available"] or 1)
-        if threads > 1 and omp_seconds > 0:
-            mlups, steps, el = timed(threads, omp_seconds)
-            out["openmp"] = dict(value=round(mlups, 3), unit="MLUPS", cores=threads,
-                                 sample=f"same tank, collision and moments on {threads} OpenMP threads, push, "
-                                        f"bounce-back and forcing serial: {steps} steps in {el:.1f} s")
+        if threads > 1:
+            c1t, el1t = timed(60, 20, threads, 7300.0)
+            out["openmp"] = dict(value=round(c1t, 3), unit="MLUPS", cores=threads,
+                                 sample=f"the 60^3 run with collision and moments on {threads} OpenMP threads, "
+                                        f"push, bounce-back and forcing serial: 20 steps in {el1t:.2f} s")
     finally:
         O.set_threads(1)
     return out
@@ -210,8 +218,9 @@ def run_batch_workload(td, hip, n_tanks, steps, precision, streaming, layout, so
     scells = sum(s["edge"] ** 3 for s in sample)
     for cu in tanks:
         cu.close()
-    solo_s = 0.0
+    solo_s, graph_s = 0.0, 0.0
     for s in sample:
+        # plain: tdlb_cu_run, the launches of each step issued from the host
         cu = make_tank(td, s, precision, streaming, layout)
         cu.run(2)
         hip.device_sync()
@@ -220,10 +229,24 @@ def run_batch_workload(td, hip, n_tanks, steps, precision, streaming, layout, so
         hip.device_sync()
         solo_s += time.time() - t0
         cu.close()
+        # graph: the same tank alone in a batch (its own tuned tile, steps replayed from a HIP graph)
+        cu = make_tank(td, s, precision, streaming, layout)
+        b1 = td.Batch(precision, layout, streaming)
+        b1.add(cu, s["steps"])
+        b1.run(2)
+        hip.device_sync()
+        t0 = time.time()
+        b1.run(done)
+        hip.device_sync()
+        graph_s += time.time() - t0
+        b1.close()
+        cu.close()
     solo = scells * done / solo_s / 1e6
+    solo_graph = scells * done / graph_s / 1e6
     return dict(tanks=n_tanks, steps=done, cells=cells, batched_mlups=batched, batched_s=el, solo_mlups=solo,
-                solo_sample_tanks=len(sample), speedup=batched / solo, setup_s=round(setup, 1),
-                autotune_s=round(tune, 2), launches_per_step=info["launches_per_step"],
+                solo_graph_mlups=solo_graph, solo_sample_tanks=len(sample), speedup=batched / solo,
+                speedup_vs_graph=batched / solo_graph, setup_s=round(setup, 1), autotune_s=round(tune, 2),
+                launches_per_step=info["launches_per_step"], block_size=info.get("block_size"),
                 tiles={int(e): t for e, t in zip([s["edge"] for s in specs], info["tiles"])})
 
 
@@ -243,21 +266,25 @@ def serve_requests(duration, rate=4.0, seed=1):
                         steps=int(rng.integers(500, 5001))))
 
 
-def run_serve_workload(td, duration, rate, tick_steps, max_resident, seed=1):
-    """Feed the arrivals to a tdlb server in real time for `duration` seconds;
-    latency = finish - submit of the requests completed inside the window."""
+def run_serve_workload(td, duration, rate, tick_steps, max_resident, seed=1, drain_s=600.0):
+    """Feed the arrivals to a tdlb server in real time for `duration` seconds,
+    then keep ticking, with no new arrivals, until every request has finished or
+    `drain_s` more seconds have passed.  Latency = finish - submit; a request
+    still unfinished at the end counts with end - submit, a lower bound
+    (censored), so p50/p99 cover every request submitted in the window."""
     import numpy as np
     reqs = serve_requests(duration, rate, seed)
     s = td.Server(td.PREC_F16, td.LAYOUT_LIJK, td.STREAM_ESOTWIST, tick_steps=tick_steps, max_resident=max_resident)
     t0 = time.time()
     i, results, ticks, last = 0, [], 0, 0.0
+    submitted = {}  # ticket -> submit time (bench clock)
     while True:
         now = time.time() - t0
         while i < len(reqs) and reqs[i]["t"] <= now:
             r = reqs[i]
-            s.submit(r["edge"], r["tip"], r["re"], r["steps"])
+            submitted[s.submit(r["edge"], r["tip"], r["re"], r["steps"])] = time.time() - t0
             i += 1
-        if now >= duration:
+        if now >= duration and (len(results) >= len(submitted) or now >= duration + drain_s):
             break
         resident, queued, n = s.tick()
         ticks += 1
@@ -268,57 +295,56 @@ def run_serve_workload(td, duration, rate, tick_steps, max_resident, seed=1):
             print(f"[serve] t={now:.0f}s submitted {i} done {len(results)} resident {resident} queued {queued}",
                   file=sys.stderr, flush=True)
             last = now
-    elapsed = time.time() - t0
+    end = time.time() - t0
     stats = s.stats()
     s.close()
     ok = [r for r in results if r["error"] == 0]
-    lat = np.array([r["finish_s"] - r["submit_s"] for r in ok]) if ok else np.array([0.0])
+    finished = {r["ticket"] for r in results}
+    lat = [r["finish_s"] - r["submit_s"] for r in ok]
+    censored = [end - t for k, t in submitted.items() if k not in finished]
+    allv = np.array(lat + censored) if (lat or censored) else np.array([0.0])
+    done_lat = np.array(lat) if lat else np.array([0.0])
     updates = sum(r["edge"] ** 3 * r["steps"] for r in ok)
-    return dict(duration_s=round(elapsed, 1), rate_per_s=rate, submitted=i, completed=len(ok),
-                failed=len(results) - len(ok), backlog=i - len(results), ticks=ticks, tick_steps=tick_steps,
-                max_resident=max_resident, p50_latency_s=round(float(np.percentile(lat, 50)), 2),
-                p99_latency_s=round(float(np.percentile(lat, 99)), 2), aggregate_mlups=updates / elapsed / 1e6,
+    offered = sum(r["edge"] ** 3 * r["steps"] for r in reqs) / max(duration, 1e-9) / 1e6
+    return dict(arrivals_s=duration, duration_s=round(end, 1), rate_per_s=rate, submitted=len(submitted),
+                completed=len(ok), failed=len(results) - len(ok), unfinished=len(censored), ticks=ticks,
+                tick_steps=tick_steps, max_resident=max_resident,
+                p50_latency_s=round(float(np.percentile(allv, 50)), 2),
+                p99_latency_s=round(float(np.percentile(allv, 99)), 2),
+                p50_latency_completed_s=round(float(np.percentile(done_lat, 50)), 2),
+                p99_latency_completed_s=round(float(np.percentile(done_lat, 99)), 2),
+                offered_mlups=round(offered, 1), aggregate_mlups=updates / end / 1e6,
                 budget_gb=round(stats["budget_bytes"] / 1e9, 1))
 
 
-def copy_calibration(gib):
-    """A plain device copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel
-    give the factor between those counters and bytes moved."""
-    import torch
-    n = int(gib * (1 << 30)) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    t0 = time.time()
-    for _ in range(5):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    el = time.time() - t0
-    print(json.dumps({"copy_bytes_each_way": n * 4, "copies": 5, "gbs": round(2 * 5 * n * 4 / el / 1e9, 1)}),
-          flush=True)
+def sol_probe(grid, elem_bytes=4, planes=19, passes=10):
+    """Speed of light of the fused kernel's access pattern (tdlb_probe_planes):
+    `planes` planes of grid^3 values, as far apart as the planes of the LIJK
+    lattice, each value read and written back in place per pass, with and
+    without non-temporal hints."""
+    import tdlbcpp_amd as td
+    cells = grid ** 3
+    stride = td.lijk_plane_bytes(grid, elem_bytes)
+    out = {"planes": planes, "elem_bytes": elem_bytes, "cells": cells, "plane_stride_bytes": stride}
+    for nt in (True, False):
+        ms = td.probe_planes(elem_bytes, cells, planes, nontemporal=nt, passes=passes, stride_bytes=stride)
+        out["nontemporal" if nt else "plain"] = {"ms": round(ms, 4),
+                                                 "gbs": round(2 * planes * cells * elem_bytes / (ms / 1e3) / 1e9, 1)}
+    out["gbs"] = max(out["nontemporal"]["gbs"], out["plain"]["gbs"])
+    return out
+
+
+def probe_calibration(gib):
+    """One plane of `gib` GiB of 4-byte values read and written back in place,
+    2 + 5 passes of k_probe_planes: the known-size kernel against which the
+    FETCH_SIZE / WRITE_SIZE counters are scaled."""
+    import tdlbcpp_amd as td
+    cells = int(gib * (1 << 30)) // 4
+    ms = td.probe_planes(4, cells, 1, nontemporal=False, passes=5)
+    print(json.dumps({"kernel": "k_probe_planes<uint32_t,false>", "bytes_read_per_dispatch": cells * 4,
+                      "bytes_written_per_dispatch": cells * 4, "dispatches": 7, "ms_per_pass": round(ms, 4),
+                      "gbs": round(2 * cells * 4 / (ms / 1e3) / 1e9, 1)}), flush=True)
     return 0
-
-
-def measured_copy_gbs(gib=2.0, reps=20):
-    """Bytes read + written per second of torch's device copy of `gib` GiB,
-    timed with events on torch's stream: the bandwidth a plain streaming
-    kernel reaches on this GPU, beside the 8 TB/s peak."""
-    import torch
-    n = int(gib * (1 << 30)) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        b.copy_(a)
-    e1.record()
-    e1.synchronize()
-    gbs = 2 * reps * n * 4 / (e0.elapsed_time(e1) / 1e3) / 1e9
-    del a, b
-    torch.cuda.empty_cache()
-    return gbs
 
 
 def main():
@@ -329,10 +355,16 @@ def main():
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--variants", default="all", help="all, precision (fp32 + fp16/bf16/e3m12 Esotwist/LIJK) or headline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-omp-seconds", type=float, default=8.0,
-                    help="seconds of the OpenMP leg of the CPU baseline (0 skips it)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0,
-                    help="seconds of oracle main-loop steps timed for cpu_baseline")
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="main-loop steps of the CPU oracle on the headline tank (cpu_baseline)")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="tank: skip the accuracy object (16-bit storage against the fp64 build, 96^3 x 5000 steps)")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="tank: skip the serving object (a short Poisson stream below capacity, drained)")
+    ap.add_argument("--serving-duration", type=float, default=30.0,
+                    help="tank: seconds of arrivals of the serving object")
+    ap.add_argument("--serving-rate", type=float, default=1.5,
+                    help="tank: arrivals per second of the serving object")
     ap.add_argument("--workload", default="tank", choices=["tank", "batch", "serve", "accuracy"],
                     help="tank: one 512^3 tank (headline); batch: independent tanks batched vs one at a time; "
                          "serve: a Poisson stream of tank requests; accuracy: fp32/fp16/bf16/e3m12 against the fp64 "
@@ -352,7 +384,10 @@ def main():
     ap.add_argument("--solo-sample", type=int, default=0,
                     help="time one-at-a-time runs on the first N tanks only (0: all)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
-                    help="only copy GIB GiB device-to-device 5 times (reference kernel for HBM byte counters)")
+                    help="only run the access-pattern probe over one plane of GIB GiB (read and written back in "
+                         "place, 7 passes): the known-size kernel for the HBM byte counters")
+    ap.add_argument("--drain", type=float, default=600.0,
+                    help="serve: seconds to keep ticking after the arrivals end (unfinished requests are censored)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -361,7 +396,7 @@ def main():
               file=sys.stderr)
         return 2
     if args.copy_calibration > 0:
-        return copy_calibration(args.copy_calibration)
+        return probe_calibration(args.copy_calibration)
 
     import tdlbcpp_amd as td
     hip = Hip()
@@ -396,7 +431,7 @@ def main():
               flush=True)
         return 0
     if args.workload == "serve":
-        r = run_serve_workload(td, args.duration, args.rate, args.tick_steps, args.max_resident)
+        r = run_serve_workload(td, args.duration, args.rate, args.tick_steps, args.max_resident, drain_s=args.drain)
         print(f"[bench] serve: p50 {r['p50_latency_s']} s, p99 {r['p99_latency_s']} s, "
               f"{r['aggregate_mlups']:.0f} MLUPS, {r['completed']}/{r['submitted']} done",
               file=sys.stderr, flush=True)
@@ -451,11 +486,36 @@ def main():
                      "kernel": "k_esotwist<float,LIJK,FUSED>",
                      "bytes_per_cell": bytes_per_cell(), "cells_per_launch": args.grid ** 3},
     }
-    copy = measured_copy_gbs()
-    out["roofline"]["copy_gbs"] = round(copy, 1)
-    out["roofline"]["frac_of_copy"] = round(achieved / copy, 4)
+    probe = sol_probe(args.grid)
+    out["roofline"]["copy_gbs"] = probe["gbs"]
+    out["roofline"]["frac_of_copy"] = round(achieved / probe["gbs"], 4)
+    out["roofline"]["sol_probe"] = probe
+    if not args.no_accuracy:
+        from tdlbcpp_amd import precision as P
+        names = {"fp16": "Esotwist/LIJK/fp16", "bf16": "Esotwist/LIJK/bf16", "e3m12": "Esotwist/LIJK/e3m12"}
+        r = P.precision_report(T=96, steps=5000, precisions=(td.PREC_F32, td.PREC_F16, td.PREC_BF16, td.PREC_E3M12),
+                               re=1.0e5, samples=20)
+        acc = {"grid": 96, "steps": 5000, "reynolds": 1.0e5, "against": "fp64 build of the same kernels",
+               "bar_velocity_l2": 1e-3, "formats": {}}
+        for k, v in r["variants"].items():
+            e = {"velocity_l2": v["velocity_l2"], "velocity_l2_mean_field": v["velocity_l2_mean"],
+                 "po_rel_err": v["po_rel_err"], "mass_drift": v["mass_drift"],
+                 "meets_bar": v["velocity_l2"] <= 1e-3, "meets_bar_mean_field": v["velocity_l2_mean"] <= 1e-3}
+            if k in names and names[k] in results:
+                e["store_speedup"] = round(results[names[k]]["mlups"] / head["mlups"], 3)
+            acc["formats"][k] = e
+        out["accuracy"] = acc
+        print("[bench] accuracy 96^3 x 5000: " + ", ".join(
+            f"{k} L2 {v['velocity_l2']:.2e} (mean field {v['velocity_l2_mean_field']:.2e})"
+            for k, v in acc["formats"].items()), file=sys.stderr, flush=True)
+    if not args.no_serving:
+        r = run_serve_workload(td, args.serving_duration, args.serving_rate, args.tick_steps, 0, drain_s=120.0)
+        out["serving"] = r
+        print(f"[bench] serving: p50 {r['p50_latency_s']} s, p99 {r['p99_latency_s']} s over {r['submitted']} "
+              f"requests ({r['unfinished']} unfinished), {r['aggregate_mlups']:.0f} MLUPS", file=sys.stderr,
+              flush=True)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(max_seconds=args.cpu_seconds, omp_seconds=args.cpu_omp_seconds)
+        out["cpu_baseline"] = cpu_baseline(grid=args.grid, steps=args.cpu_steps)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -119,6 +119,8 @@ def lib():
             "tdlb_cu_profile_read": ([vp, P(d), P(i64), P(d), P(i64)], i32),
             "tdlb_cu_reductions": ([vp, P(d)], i32),
             "tdlb_cu_reductions_noalloc": ([vp, vp], i32),
+            "tdlb_probe_planes": ([ctypes.c_int32, ctypes.c_int32, i64, i64, ctypes.c_int32, ctypes.c_int32,
+                                   ctypes.c_int32, P(d)], i32),
             "tdlb_cu_macroscopic": ([vp, vp, ctypes.c_size_t], i32),
             "tdlb_cu_double_resolution": ([vp], i32),
             "tdlb_cu_output_exclude": ([vp, vp, ctypes.c_size_t], i32),
@@ -160,6 +162,28 @@ def device_count():
     n = ctypes.c_int()
     check(lib().tdlb_device_count(ctypes.byref(n)))
     return n.value
+
+
+def probe_planes(elem_bytes, cells, planes, nontemporal=True, passes=10, device=0, stride_bytes=0):
+    """Milliseconds per pass of the access-pattern probe (tdlb_probe_planes): every value of `planes`
+    planes of `cells` values, stride_bytes apart (0: rounded up to 1 MiB), read and written back in place;
+    bytes per pass 2 * planes * cells * elem_bytes."""
+    ms = ctypes.c_double()
+    check(lib().tdlb_probe_planes(device, elem_bytes, cells, stride_bytes, planes, 1 if nontemporal else 0, passes,
+                                  ctypes.byref(ms)))
+    return ms.value
+
+
+def lijk_plane_bytes(grid, elem_bytes):
+    """Bytes between the population planes of a grid^3 LIJK lattice in HBM: the interior k-rows of every
+    (i, j) row including the ghost rows, rounded to 128 B lines, then the two k-ghost cells of every row,
+    the whole rounded up to 1 MiB when it reaches 16 MiB (else to a 128 B line)."""
+    xg = yg = grid + 2
+    line = 128 // elem_bytes
+    side = (xg * yg * grid + line - 1) // line * line
+    cells = side + 2 * xg * yg
+    align = (1 << 20) // elem_bytes if cells * elem_bytes >= (16 << 20) else line
+    return (cells + align - 1) // align * align * elem_bytes
 
 
 def make_flow(**kw):

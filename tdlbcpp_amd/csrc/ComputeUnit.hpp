@@ -1,6 +1,8 @@
 // C++ host-side ComputeUnit over the C ABI: the reference's ComputeUnitBase
 // surface (tdlbcpp/src/ComputeUnit.h:39-204) backed by a handle in HBM, plus the
-// checkpoint naming of DiskOutputTree (tdlbcpp/src/DiskOutputTree.h:418-590).
+// checkpoint and plot naming of DiskOutputTree (tdlbcpp/src/DiskOutputTree.h:
+// 331-410, 418-590) and the plane and vorticity outputs of
+// ComputeUnitOutput.hpp:113-245, 354-447.
 //
 // The reference selects layout, streaming and precision with template
 // arguments; here they are constructor arguments because the device code is
@@ -10,13 +12,17 @@
 
 #include <sys/stat.h>
 
+#include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <ctime>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/tdlb.h"
+#include "jpeg_gray.hpp"
 #include "params.hpp"
 
 namespace tdlb {
@@ -72,6 +78,52 @@ public:
     }
     std::string getAllParamsFilePath(const std::string& dir, const std::string& unit) const {
         return dir + "/AllParams." + formatId() + "." + unit;
+    }
+
+    // ---- plot naming (DiskOutputTree.h:331-410): <outputRootDir>_<initTime>/
+    // <prefix>.<plotType>.V5.step_<8-digit step>[.cut_<n>], files CUid.<id>.V5.*
+    std::string outputDir() const {
+        std::string root = "Error_UnInitialised_Output_Dir";
+        if (output_.contains("outputRootDir")) root = output_.at("outputRootDir").get<std::string>();
+        return root + "_" + initTime;
+    }
+    static std::string formatStep(tStep step) {
+        char buf[32];
+        std::snprintf(buf, sizeof buf, "%08lu", (unsigned long)step);
+        return buf;
+    }
+    std::string formatPlotDir(const std::string& prefix, const std::string& plotType, tStep step) const {
+        return outputDir() + "/" + prefix + "." + plotType + ".V5.step_" + formatStep(step);
+    }
+    std::string formatXYPlaneDir(tStep step, tNi atK, const std::string& prefix = "plot") const {
+        return formatPlotDir(prefix, "xyPlane", step) + ".cut_" + std::to_string(atK);
+    }
+    std::string formatXZPlaneDir(tStep step, tNi atJ, const std::string& prefix = "plot") const {
+        return formatPlotDir(prefix, "xzPlane", step) + ".cut_" + std::to_string(atJ);
+    }
+    std::string formatCUid() const { return "CUid." + formatId(); }
+    std::string formatQVecBinFileNamePath(const std::string& dir) const {
+        return dir + "/" + formatCUid() + ".V5.QVec.bin";
+    }
+    std::string formatF3BinFileNamePath(const std::string& dir) const {
+        return dir + "/" + formatCUid() + ".V5.F3.bin";
+    }
+    // <bin.filePath>.json holding every parameter block (writeAllParamsJson)
+    void writeAllParamsJson(const BinFileParams& bin, const RunningParams& run) const {
+        const std::string path = bin.filePath + ".json";
+        FILE* fp = std::fopen(path.c_str(), "w");
+        if (!fp) throw std::runtime_error("cannot write " + path);
+        const std::string text = getJsonParams(bin, run).dump();
+        const bool ok = std::fwrite(text.data(), 1, text.size(), fp) == text.size();
+        if (std::fclose(fp) != 0 || !ok) throw std::runtime_error("cannot write " + path);
+    }
+    // mkdir -p
+    static void createDir(const std::string& path) {
+        for (size_t p = path.find('/', 1); ; p = path.find('/', p + 1)) {
+            const std::string d = p == std::string::npos ? path : path.substr(0, p);
+            if (!d.empty()) ::mkdir(d.c_str(), 0775);
+            if (p == std::string::npos) break;
+        }
     }
     json::Value getJsonParams(const BinFileParams& bin, const RunningParams& run) const {
         json::Value j = json::Value::object();
@@ -205,6 +257,62 @@ public:
     std::string lastCheckpointDir;
     int missingArrays = 0;
 
+    // ---- outputs (ComputeUnitOutput.hpp, main.cpp:466-487)
+    // ExcludeOutputPoints: outside the tank and every cell holding a point of
+    // the bound geometry (setOutputExcludePoints(geomFORCING))
+    std::vector<uint8_t> ExcludeOutputPoints;
+    void setOutputExcludePoints() {
+        ExcludeOutputPoints.assign(size, 0);
+        throw_if(tdlb_cu_output_exclude(h_, ExcludeOutputPoints.data(), size), "setOutputExcludePoints");
+    }
+    // the same from host lists (a ComputeUnit driven by forcing() calls, with
+    // no geometry bound): the tank's solid mask (main.cpp:309-315), then every
+    // point of geom at indexPlusGhost (ComputeUnitOutput.hpp:29-34)
+    void setOutputExcludePoints(const tdlb_rushton_params& rp, const std::vector<PosPolar>& geom) {
+        ExcludeOutputPoints.assign(size, 0);
+        throw_if(tdlb_rushton_solid_mask(&rp, xg, yg, zg, ghost, ExcludeOutputPoints.data()),
+                 "setOutputExcludePoints");
+        for (const PosPolar& p : geom)
+            ExcludeOutputPoints[((size_t)(p.i + ghost) * yg + (size_t)(p.j + ghost)) * zg + (size_t)(p.k + ghost)] = 1;
+    }
+    void unsetOutputExcludePoints() { ExcludeOutputPoints.clear(); }
+
+    // savePlaneXZ<float, 4> (ComputeUnitOutput.hpp:354-447): tDiskGrid<float,4>
+    // records of the plane j = cutAt and tDiskGrid<float,3> records of its
+    // non-zero forces, each file with its AllParams JSON.  The 2 alignment
+    // bytes after the coordinates of every record are written as zero (the
+    // reference leaves them unset).
+    void savePlaneXZ(tNi cutAt, BinFileParams bin, const RunningParams& run) {
+        const size_t cells = (size_t)xg1 * zg1;
+        std::vector<uint8_t> qv(cells * 24, 0), f3(cells * 20, 0);
+        int64_t nq = 0, nf = 0;
+        throw_if(tdlb_cu_plane_xz(h_, cutAt, exclude(), qv.data(), &nq, f3.data(), &nf), "savePlaneXZ");
+        const std::string dir = tree_->formatXZPlaneDir(run.step, cutAt);
+        DiskOutputTree::createDir(dir);
+        bin.filePath = tree_->formatQVecBinFileNamePath(dir);
+        bin.binFileSizeInStructs = (tNi)nq;
+        std::cout << "Writing output to: " << bin.filePath << std::endl;
+        tree_->writeAllParamsJson(bin, run);
+        write_file(bin.filePath, qv.data(), (size_t)nq * 24);
+        bin.QOutputLength = 3;
+        bin.binFileSizeInStructs = (tNi)nf;
+        bin.filePath = tree_->formatF3BinFileNamePath(dir);
+        tree_->writeAllParamsJson(bin, run);
+        write_file(bin.filePath, f3.data(), (size_t)nf * 20);
+    }
+
+    // calcVorticityXZ / calcVorticityXY (ComputeUnitOutput.hpp:113-245): the
+    // picture floor(255 (v - min) / (max - min)) of v = log|curl u|^2 on the
+    // plane, min and max over its finite values, as a JPEG of quality 100 named
+    // by the plane's plot directory.  Excluded and non-finite cells are 0 (the
+    // reference leaves excluded cells unset and converts -inf out of range).
+    void calcVorticityXZ(tNi j, const RunningParams& run, int /*jpegCompression: quality 100 as the reference*/) {
+        vorticity_picture(0, j, tree_->formatXZPlaneDir(run.step, j) + ".jpeg");
+    }
+    void calcVorticityXY(tNi k, const RunningParams& run, int /*jpegCompression*/) {
+        vorticity_picture(1, k, tree_->formatXYPlaneDir(run.step, k) + ".jpeg");
+    }
+
     static tdlb_flow to_c(const FlowParams& f) {
         tdlb_flow c{};
         c.initial_rho = f.initialRho; c.re_m_nondimensional = f.reMNonDimensional; c.uav = f.uav;
@@ -218,6 +326,52 @@ public:
 private:
     tdlb_cu_t h_ = nullptr;
     DiskOutputTree* tree_;
+
+    const uint8_t* exclude() const { return ExcludeOutputPoints.empty() ? nullptr : ExcludeOutputPoints.data(); }
+
+    static void write_file(const std::string& path, const void* data, size_t bytes) {
+        FILE* fp = std::fopen(path.c_str(), "wb");
+        if (!fp) throw std::runtime_error("cannot write " + path);
+        const bool ok = bytes == 0 || std::fwrite(data, 1, bytes, fp) == bytes;
+        if (std::fclose(fp) != 0 || !ok) throw std::runtime_error("cannot write " + path);
+    }
+
+    template <typename T>
+    std::vector<uint8_t> picture(int axis, tNi cut) const {
+        const tNi n2 = axis == 0 ? zg1 : yg1;
+        std::vector<T> v((size_t)xg1 * n2);
+        throw_if(tdlb_cu_vorticity(h_, axis, cut, exclude(), v.data(), v.size() * sizeof(T)), "vorticity");
+        bool have = false;
+        T lo = 0, hi = 0;
+        for (T a : v)
+            if (std::isfinite(a)) {
+                if (!have || a < lo) lo = a;
+                if (!have || a > hi) hi = a;
+                have = true;
+            }
+        // XZ: pict[zg1 (i-1) + (k-1)] of a zg1-row, xg1-wide picture; XY:
+        // pict[(i-1) + xg1 (j-1)] of a yg1-row picture
+        std::vector<uint8_t> pict((size_t)xg1 * n2, 0);
+        for (tNi i = 0; i < xg1; i++)
+            for (tNi b = 0; b < n2; b++) {
+                const T a = v[(size_t)i * n2 + b];
+                uint8_t px = 0;
+                if (std::isfinite(a)) {
+                    const T s = std::floor(T(255) * ((a - lo) / (hi - lo)));
+                    if (std::isfinite(s)) px = (uint8_t)(s < T(0) ? T(0) : s > T(255) ? T(255) : s);
+                }
+                pict[axis == 0 ? (size_t)i * n2 + b : (size_t)i + (size_t)xg1 * b] = px;
+            }
+        return pict;
+    }
+
+    void vorticity_picture(int axis, tNi cut, const std::string& path) const {
+        const std::vector<uint8_t> pict = precision == TDLB_PREC_F64 ? picture<double>(axis, cut)
+                                                                     : picture<float>(axis, cut);
+        DiskOutputTree::createDir(tree_->outputDir());
+        if (!jpeg::write_gray(path, pict.data(), (int)xg1, (int)(axis == 0 ? zg1 : yg1), 100))
+            throw std::runtime_error("cannot write " + path);
+    }
 
     void init(const ComputeUnitParams& cu) {
         idi = cu.idi; idj = cu.idj; idk = cu.idk; nodeID = cu.nodeID; deviceID = cu.deviceID;

@@ -715,6 +715,48 @@ int tdlb_device_count(int* count) {
     return TDLB_OK;
 }
 
+int tdlb_probe_planes(int32_t device_id, int32_t elem_bytes, int64_t cells, int64_t plane_stride_bytes,
+                      int32_t planes, int32_t nontemporal, int32_t passes, double* ms_per_pass) {
+    if (!ms_per_pass) return fail(TDLB_ERR_INVALID, "null argument");
+    if (elem_bytes != 2 && elem_bytes != 4 && elem_bytes != 8) return fail(TDLB_ERR_INVALID, "elem_bytes: 2, 4 or 8");
+    if (cells < 1 || planes < 1 || planes > 20 || passes < 1)
+        return fail(TDLB_ERR_INVALID, "cells >= 1, planes 1..20, passes >= 1");
+    if ((uint64_t)cells > (1ull << 40)) return fail(TDLB_ERR_INVALID, "cells too large");
+    const uint64_t align = 1ull << 20;
+    // default stride: the plane rounded up to 1 MiB
+    const uint64_t plane_bytes = plane_stride_bytes > 0 ? (uint64_t)plane_stride_bytes
+                                                        : ((uint64_t)cells * elem_bytes + align - 1) / align * align;
+    if (plane_bytes < (uint64_t)cells * elem_bytes || plane_bytes % 256 != 0)
+        return fail(TDLB_ERR_INVALID, "plane_stride_bytes: at least cells * elem_bytes, a multiple of 256");
+    HIPCHK(hipSetDevice(device_id < 0 ? 0 : device_id));
+    void* base = nullptr;
+    if (hipMalloc(&base, plane_bytes * planes) != hipSuccess) return fail(TDLB_ERR_NOMEM, "probe planes");
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_zero(base, plane_bytes * planes, s);
+    const uint64_t stride = plane_bytes / elem_bytes;
+    for (int w = 0; w < 2 && e == hipSuccess; w++)
+        e = launch_probe_planes(elem_bytes, base, stride, (uint64_t)cells, planes, nontemporal, s);
+    if (e == hipSuccess) e = hipEventRecord(e0, s);
+    for (int p = 0; p < passes && e == hipSuccess; p++)
+        e = launch_probe_planes(elem_bytes, base, stride, (uint64_t)cells, planes, nontemporal, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (s) (void)hipStreamSynchronize(s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    (void)hipFree(base);
+    if (e != hipSuccess) return fail(TDLB_ERR_HIP, std::string("probe: ") + hipGetErrorString(e));
+    *ms_per_pass = (double)ms / passes;
+    return TDLB_OK;
+}
+
 int tdlb_flow_calc_nu_re_m(tdlb_flow* f, int32_t outer_radius, int32_t precision) {
     if (!f) return fail(TDLB_ERR_INVALID, "null flow");
     if (precision == TDLB_PREC_F32 || precision == TDLB_PREC_F16 || precision == TDLB_PREC_BF16 ||

@@ -60,6 +60,11 @@ struct BladeStep {
 
 hipError_t launch_fill_q(int prec, int layout, void* Q, const Grid& g, double rho, hipStream_t s);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+// One pass of the access-pattern probe: `planes` (1..20) planes of n values of
+// elem_bytes (2, 4, 8) bytes, `stride` values apart from base, each value read
+// and written back (bits flipped) by one lane; nt: non-temporal hints.
+hipError_t launch_probe_planes(int elem_bytes, void* base, uint64_t stride, uint64_t n, int planes, int nt,
+                               hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // Populations in Q (stored type), moments in mout (arithmetic type, Q's shape

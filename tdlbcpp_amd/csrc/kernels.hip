@@ -1905,6 +1905,45 @@ hipError_t launch_q_boundary(int esize, int dlayout, int blayout, int to_boundar
     return hipErrorInvalidValue;
 }
 
+// Speed of light of the stream-collide access pattern: one pass reads the
+// value of every lane's cell from each of `planes` planes, then writes each
+// back with its bits flipped by a run-time mask (so no load or store folds
+// away; two passes restore the data).  Blocks of kRowBlock lanes, as the
+// stream-collide rows.
+constexpr int kProbePlanesMax = 20;
+template <typename U, bool NT>
+__global__ __launch_bounds__(kRowBlock) void k_probe_planes(U* __restrict__ base, uint64_t stride, uint64_t n,
+                                                            int planes, U mask) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    U v[kProbePlanesMax];
+#pragma unroll
+    for (int p = 0; p < kProbePlanesMax; p++)
+        if (p < planes) v[p] = load_hint<NT>(base + (uint64_t)p * stride + c);
+#pragma unroll
+    for (int p = 0; p < kProbePlanesMax; p++)
+        if (p < planes) store_hint<NT>(base + (uint64_t)p * stride + c, (U)(v[p] ^ mask));
+}
+
+template <typename U>
+hipError_t probe_t(void* base, uint64_t stride, uint64_t n, int planes, int nt, hipStream_t s) {
+    const uint32_t nb = (uint32_t)((n + kRowBlock - 1) / kRowBlock);
+    const U mask = (U)0x5555555555555555ull;
+    if (nt) hipLaunchKernelGGL((k_probe_planes<U, true>), dim3(nb), dim3(kRowBlock), 0, s, (U*)base, stride, n, planes, mask);
+    else hipLaunchKernelGGL((k_probe_planes<U, false>), dim3(nb), dim3(kRowBlock), 0, s, (U*)base, stride, n, planes, mask);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_planes(int elem_bytes, void* base, uint64_t stride, uint64_t n, int planes, int nt,
+                               hipStream_t s) {
+    if (planes < 1 || planes > kProbePlanesMax || n == 0 || (n + kRowBlock - 1) / kRowBlock > 0x7fffffffull)
+        return hipErrorInvalidValue;
+    if (elem_bytes == 2) return probe_t<uint16_t>(base, stride, n, planes, nt, s);
+    if (elem_bytes == 4) return probe_t<uint32_t>(base, stride, n, planes, nt, s);
+    if (elem_bytes == 8) return probe_t<uint64_t>(base, stride, n, planes, nt, s);
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
     if (bytes == 0) return hipSuccess;
     if (bytes % 4 == 0 && ((uintptr_t)p % 4) == 0)

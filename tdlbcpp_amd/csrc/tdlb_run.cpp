@@ -1,15 +1,19 @@
 // tdlb_run -- the reference's main loop (tdlbcpp/src/main.cpp:55-555) on one MI355X.
 //
 //   tdlb_run -i input.json [-c checkpoint_dir] [--layout IJKL|LIJK] [--precision f32|f64]
-//            [--per-call] [--steps N] [--print-params] [--nu-from-json]
+//            [--per-call] [--steps N] [--print-params] [--nu-from-json] [--write-planes] [--no-output]
 //
 // Reads the reference's input JSON, builds one ComputeUnit (ghost = multiStep),
 // initialises it or loads a checkpoint, generates the Rushton tank, sets nu from
 // calcNuAndRe_m, runs the initial forcing and then num_steps iterations of
 // collision -> bounceBack -> streamingPush -> moments -> forcing, writing
-// checkpoints every checkpointRepeat steps.  The default path runs the fused
-// device loop (tdlb_cu_run); --per-call drives the five reference calls per
-// step with host-generated IB lists, as main.cpp does.
+// checkpoints every checkpointRepeat steps and, as main.cpp:462-487 does after
+// every step, the vorticity pictures of OutputParams' XY_vorticity_planes and
+// XZ_vorticity_planes that are due.  --write-planes also writes the
+// savePlaneXZ records of XZ_planes (main.cpp:478-482 has that call commented
+// out); --no-output writes no plots.  The default path runs the fused device
+// loop (tdlb_cu_run); --per-call drives the five reference calls per step with
+// host-generated IB lists, as main.cpp does.
 // --print-params parses the input and prints the parsed parameters (no GPU).
 #include <chrono>
 #include <cmath>
@@ -28,7 +32,8 @@ namespace {
 void usage() {
     std::fprintf(stderr,
                  "usage: tdlb_run -i input.json [-c checkpoint_dir] [--layout IJKL|LIJK] [--precision f32|f64]\n"
-                 "                [--per-call] [--steps N] [--print-params] [--nu-from-json]\n");
+                 "                [--per-call] [--steps N] [--print-params] [--nu-from-json] [--write-planes]\n"
+                 "                [--no-output]\n");
 }
 
 std::vector<tdlb_ib_point> generate(const tdlb_rushton_params& rp, int group, double angle, double delta) {
@@ -37,6 +42,32 @@ std::vector<tdlb_ib_point> generate(const tdlb_rushton_params& rp, int group, do
     std::vector<tdlb_ib_point> v(n);
     throw_if(tdlb_rushton_generate(&rp, group, angle, delta, v.data(), n, &n), "geometry");
     return v;
+}
+
+// an entry of OutputParams' plane lists (OrthoPlaneParams,
+// OrthoPlaneVorticityParams): the fields the main loop reads
+struct PlaneOutput {
+    tNi cutAt = 0;
+    tStep repeat = 0, start_at_step = 0;
+    int jpegCompression = 100;
+    // main.cpp:470-485: repeat && step >= start_at_step && (step - start) % repeat == 0
+    bool due(tStep step) const {
+        return repeat && step >= start_at_step && (step - start_at_step) % repeat == 0;
+    }
+};
+
+std::vector<PlaneOutput> plane_list(const json::Value& output, const char* key) {
+    std::vector<PlaneOutput> out;
+    if (!output.contains(key)) return out;
+    for (const json::Value& e : output.at(key).elements()) {
+        PlaneOutput p;
+        p.cutAt = e.value<tNi>("cutAt", 0);
+        p.repeat = e.value<tStep>("repeat", 0);
+        p.start_at_step = e.value<tStep>("start_at_step", 0);
+        p.jpegCompression = e.value<int>("jpegCompression", 100);
+        out.push_back(p);
+    }
+    return out;
 }
 
 std::vector<PosPolar> to_pos(const std::vector<tdlb_ib_point>& v) {
@@ -51,7 +82,7 @@ std::vector<PosPolar> to_pos(const std::vector<tdlb_ib_point>& v) {
 int main(int argc, char** argv) {
     std::string input = "input_debug_gridx60_numSteps20.json", ckdir;
     int layout = TDLB_LAYOUT_IJKL, precision = TDLB_PREC_F32;
-    bool per_call = false, print_only = false, nu_from_json = false;
+    bool per_call = false, print_only = false, nu_from_json = false, write_planes = false, no_output = false;
     long long steps_override = -1;
     for (int a = 1; a < argc; a++) {
         std::string s = argv[a];
@@ -67,6 +98,8 @@ int main(int argc, char** argv) {
         else if (s == "--steps") steps_override = std::atoll(next().c_str());
         else if (s == "--print-params") print_only = true;
         else if (s == "--nu-from-json") nu_from_json = true;
+        else if (s == "--write-planes") write_planes = true;
+        else if (s == "--no-output") no_output = true;
         else if (s == "-h" || s == "--help") { usage(); return 1; }
         else { usage(); return 1; }
     }
@@ -146,6 +179,30 @@ int main(int argc, char** argv) {
         std::cout << "ComputeUnit " << lb.x << "x" << lb.y << "x" << lb.z << " streaming=" << flow.streaming
                   << " layout=" << cu.strMemoryLayout << " nu=" << flow.nu << " LES=" << flow.useLES << std::endl;
 
+        std::vector<PosPolar> geomFORCING;  // per-call: the lists of the last forcing pass
+        const std::vector<PlaneOutput> xy_vort = plane_list(output, "XY_vorticity_planes");
+        const std::vector<PlaneOutput> xz_vort = plane_list(output, "XZ_vorticity_planes");
+        const std::vector<PlaneOutput> xz_planes = plane_list(output, "XZ_planes");
+        // main.cpp:462-487 after each step: exclusion from the forcing geometry,
+        // then the vorticity pictures (and, on request, the XZ plane records) due
+        auto write_output = [&]() {
+            if (no_output) return;
+            bool any = false;
+            for (auto* l : {&xy_vort, &xz_vort}) for (const PlaneOutput& p : *l) any |= p.due(running.step);
+            if (write_planes) for (const PlaneOutput& p : xz_planes) any |= p.due(running.step);
+            if (!any) return;
+            if (per_call) lb.setOutputExcludePoints(rp, geomFORCING);
+            else lb.setOutputExcludePoints();
+            for (const PlaneOutput& p : xy_vort)
+                if (p.due(running.step)) lb.calcVorticityXY(p.cutAt, running, p.jpegCompression);
+            if (write_planes)
+                for (const PlaneOutput& p : xz_planes)
+                    if (p.due(running.step)) lb.savePlaneXZ(p.cutAt, binFormat, running);
+            for (const PlaneOutput& p : xz_vort)
+                if (p.due(running.step)) lb.calcVorticityXZ(p.cutAt, running, p.jpegCompression);
+            lb.unsetOutputExcludePoints();
+        };
+
         const bool rnd32 = precision == TDLB_PREC_F32;
         auto inc = [&](tStep s) {
             double d = 0;
@@ -223,7 +280,9 @@ int main(int argc, char** argv) {
                 lb.streamingPush();
                 lb.moments();
                 lb.forcing(all, flow.alpha, flow.beta);
+                geomFORCING.swap(all);
             }
+            write_output();
             if (checkpoint.checkpointRepeat && (running.step % checkpoint.checkpointRepeat == 0))
                 lb.checkpoint_write("device", running, binFormat);
             if (step == first || step == running.num_steps || (step % 10) == 0) {

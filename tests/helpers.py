@@ -42,6 +42,7 @@ class ReferenceLoop:
             self.cu.bounce_back()
             self.cu.streaming_push()
             self.cu.moments()
+            self.rot = rot  # the rotating list of the last step (output exclusion)
             self.cu.forcing(np.concatenate([self.fixed, self.nonupd, rot]), self.alfa, self.beta)
 
 

@@ -117,18 +117,75 @@ def _config1_input(tmp_path):
     j["RunningParams"].update(num_steps=20, step=1, angle=0.0, doubleResolutionAtStep=1000)
     j["CheckpointParams"].update(checkpointRepeat=10, checkpointWriteRootDir=str(tmp_path),
                                  checkpointWriteDirAppendTime=False, checkpointWriteDirPrefix="cfg1")
+    j["OutputParams"]["outputRootDir"] = str(tmp_path / "out")
     p = tmp_path / "input.json"
     p.write_text(json.dumps(j))
     return p, j
 
 
+def _picture(v):
+    """The 8-bit picture of a vorticity plane as the driver forms it, in float:
+    floor(255 (v - min) / (max - min)) over the finite values, 0 elsewhere."""
+    v = np.asarray(v, dtype=np.float32)
+    fin = np.isfinite(v)
+    lo, hi = v[fin].min(), v[fin].max()
+    with np.errstate(invalid="ignore", divide="ignore"):
+        s = np.floor(np.float32(255) * ((v - lo) / (hi - lo)))
+    return np.where(fin & np.isfinite(s), np.clip(s, 0, 255), 0).astype(np.int32)
+
+
+def _check_outputs(tmp_path, o, ref, T):
+    """The step-20 outputs of the 60^3 input (main.cpp:462-487 and the XZ_planes
+    records of --write-planes) against the oracle's records and vorticity."""
+    from PIL import Image
+    outs = glob.glob(str(tmp_path / "out_*"))
+    assert len(outs) == 1, outs
+    od = outs[0]
+    names = sorted(os.listdir(od))
+    for step in ("00000010", "00000020"):
+        assert f"plot.xyPlane.V5.step_{step}.cut_30.jpeg" in names
+        assert f"plot.xzPlane.V5.step_{step}.cut_40.jpeg" in names
+        assert f"plot.xzPlane.V5.step_{step}.cut_40" in names
+    rp = td.rushton_defaults(T, tip_speed=0.1, startup_steps=30)
+    xg = T + 2
+    ex = td.rushton_solid_mask(rp, xg, xg, xg).copy().reshape(xg, xg, xg)
+    for pts in (ref.fixed, ref.nonupd, ref.rot):
+        ex[pts["i"] + 1, pts["j"] + 1, pts["k"] + 1] = 1
+    ex = ex.reshape(-1)
+    pdir = os.path.join(od, "plot.xzPlane.V5.step_00000020.cut_40")
+    oq, of = o.plane_xz(40, ex)
+    for fname, want, qlen in (("CUid.0.0.0.V5.QVec.bin", oq, 4), ("CUid.0.0.0.V5.F3.bin", of, 3)):
+        got = np.fromfile(os.path.join(pdir, fname), dtype=want.dtype)
+        assert len(got) == len(want) > 0
+        for name in ("i", "j", "k", "q"):
+            np.testing.assert_array_equal(got[name], want[name])
+        meta = json.loads(open(os.path.join(pdir, fname + ".json")).read())
+        assert meta["BinFileParams"]["binFileSizeInStructs"] == len(want)
+        assert meta["BinFileParams"]["QOutputLength"] == qlen
+        assert meta["BinFileParams"]["filePath"] == os.path.join(pdir, fname)
+        assert meta["RunningParams"]["step"] == 20
+    for axis, cut, tag in ((0, 40, "xzPlane"), (1, 30, "xyPlane")):
+        want = _picture(o.vorticity(axis, cut, ex))
+        if axis == 1:
+            want = want.T  # pict[(i-1) + x (j-1)]: rows are j
+        with Image.open(os.path.join(od, f"plot.{tag}.V5.step_00000020.cut_{cut}.jpeg")) as im:
+            assert im.format == "JPEG" and im.mode == "L" and im.size == (T, T)
+            got = np.asarray(im, dtype=np.int32)
+        # JPEG at quality 100 (at most 1 level) plus a last-place difference of
+        # the device's log that can move a floor by one level
+        err = np.abs(got - want)
+        assert err.max() <= 3 and err.mean() < 0.5, (err.max(), err.mean())
+
+
 @pytest.mark.parametrize("mode", ["fused", "per-call"])
 def test_driver_config1_matches_oracle(tmp_path, mode):
     """tdlb_run on configuration 1: the checkpoint written at running.step 20
-    (after 19 main-loop iterations) equals the oracle's Q, F and Nu to the bar."""
+    (after 19 main-loop iterations) equals the oracle's Q, F and Nu to the bar,
+    and the outputs of step 20 (vorticity pictures, XZ plane records) match the
+    oracle's."""
     case(f"driver config 1 {mode}")
     inp, j = _config1_input(tmp_path)
-    args = [os.path.join(ROOT, "tdlbcpp_amd", "tdlb_run"), "-i", str(inp)]
+    args = [os.path.join(ROOT, "tdlbcpp_amd", "tdlb_run"), "-i", str(inp), "--write-planes"]
     if mode == "per-call":
         args.append("--per-call")
     r = subprocess.run(args, capture_output=True, text=True, timeout=300)
@@ -153,6 +210,7 @@ def test_driver_config1_matches_oracle(tmp_path, mode):
     assert_close_moments(n, o.get(O.FIELD_NU), what="Nu", rho0=1e-3)
     print(f"[gpu case] config 1 bit-identical Q: {np.array_equal(q, o.get(O.FIELD_Q))}", flush=True)
     assert len(glob.glob(str(tmp_path / "cfg1_checkpoint" / "*"))) == 2
+    _check_outputs(tmp_path, o, ref, T)
 
 
 @pytest.mark.parametrize("mode", ["fused", "per-call"])

@@ -130,6 +130,45 @@ def test_16bit_storage_tracks_fp64(prec, stream):
         assert abs(drift) <= 1e-6
 
 
+# Measured at the target's workload (96^3, 5000 steps from rest, Esotwist, LIJK,
+# LES, tip speed 0.1), the runs being deterministic for a given build: velocity
+# L2 at the last step / of the mean field over the last 1250 steps, against fp64,
+# Re 1e5: fp32 7.1e-5 / 3.9e-5, fp16 5.6e-3 / 3.0e-3, bf16 3.7e-2 / 1.9e-2,
+# e3m12 2.7e-3 / 8.4e-4.  Builds that change the rounding of the 16-bit path
+# move the instantaneous values by up to 2-3x (the turbulent impeller flow
+# amplifies any storage rounding by step 5000); the mean field moves less.
+# The bar is a velocity L2 <= 1e-3.  E3M12 meets it on the mean field; no
+# 16-bit format meets it on the instantaneous field, and fp16 and bf16 miss it
+# on both, so those are held to bands around their measurements only, as
+# regression guards.
+TARGET_LIMITS = {  # (instantaneous L2, mean-field L2, |Po rel. error|)
+    "fp32": (2e-4, 1e-4, 1e-4),
+    "fp16": (2e-2, 6e-3, 1e-3),
+    "bf16": (1.5e-1, 6e-2, 1e-2),
+    "e3m12": (6e-3, 1e-3, 5e-4),
+}
+
+
+@pytest.mark.parametrize("re_", [1.0e5, 7300.0], ids=["Re1e5", "Re7300"])
+def test_storage_accuracy_at_the_target_workload(re_):
+    """96^3 x 5000 steps of every storage format against the fp64 build: E3M12
+    meets the 1e-3 velocity-L2 bar on the mean field; every format stays within
+    its measured band; the 16-bit formats conserve mass."""
+    case(f"accuracy 96^3 x 5000 Re={re_:g}")
+    r = P.precision_report(T=96, steps=5000, precisions=(td.PREC_F32, td.PREC_F16, td.PREC_BF16, td.PREC_E3M12),
+                           re=re_, samples=20)
+    for name, v in r["variants"].items():
+        print(f"[gpu case] {name}: L2 {v['velocity_l2']:.3e} mean field {v['velocity_l2_mean']:.3e} "
+              f"Po err {v['po_rel_err']:.2e} mass drift {v['mass_drift']:.2e} ({v['seconds']:.1f} s)", flush=True)
+    for name, (l2_max, mean_max, po_max) in TARGET_LIMITS.items():
+        v = r["variants"][name]
+        assert v["velocity_l2"] <= l2_max, name
+        assert v["velocity_l2_mean"] <= mean_max, name
+        assert v["po_rel_err"] <= po_max, name
+        assert abs(v["mass_drift"]) <= 1e-6, name
+    assert r["variants"]["e3m12"]["velocity_l2_mean"] <= 1e-3  # the bar
+
+
 def test_fp64_rest_tank_conserves_mass_exactly():
     """The fp64 build with the impeller at rest keeps the fluid at rest."""
     T = 36
