@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 #include "codec16.h"
@@ -355,6 +356,193 @@ __global__ __launch_bounds__(kRowBlock) void k_esotwist(typename PR::S* __restri
     const int k = rb.k0 + (int)threadIdx.x;
     if (k > g.zg1) return;
     esotwist_cell<PR, L, MODE, EVEN, HINT>(Q, mout, F, Nu, fmask, g, P, rho0, rb.i, rb.j, k);
+}
+
+// ------------------------------------------------------------------ Esotwist, IJKL through LDS
+// The fused step in IJKL, where a cell's 18 slots are one 72-byte record and a
+// lane reading single slots spans a line per lane.  A workgroup owns a tile
+// of RI x KT cells in (i, k) and marches along j through a chunk of planes,
+// holding two slabs in LDS: the records of rows i0 .. i0 + RI and cells
+// k0 .. k0 + KT (the extra row and cell are the tile's +1 halo, or the
+// periodic wrap) of plane j and of plane j + 1.  A slab is read from HBM with
+// consecutive lanes on consecutive words (whole records, coalesced), the cells
+// of plane j take their 18 values from the two slabs and put the collided ones
+// back in place, and the slab of plane j is written back, every word whose
+// owner is a cell of this tile that has run: a slot of a record belongs to one
+// cell (home(c, d) = c + twist(d), so the owner is the record - twist), and the
+// words owned by another tile, or by a plane outside the chunk, are neither
+// used nor written.  Two tiles may write disjoint words of one line; no word
+// has two writers.
+constexpr int kSlabPad = kQ + 1;  // LDS words per record: consecutive records start in different banks
+
+template <int EVEN>
+__device__ __forceinline__ uint32_t slot_twist_mask(int axis) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int s = 0; s < kQ; s++) {
+        const int d = EVEN ? s : opposite(s);
+        const int t = axis == 0 ? twist_dx(d) : axis == 1 ? twist_dy(d) : twist_dz(d);
+        m |= (uint32_t)t << s;
+    }
+    return m;
+}
+
+template <typename PR, int EVEN, int RI, int KT>
+__global__ __launch_bounds__(RI * KT) void k_esotwist_ijkl_lds(typename PR::S* __restrict__ Q,
+                                                            const typename PR::T* __restrict__ F,
+                                                            typename PR::N* __restrict__ Nu,
+                                                            const uint64_t* __restrict__ fmask, Grid g,
+                                                            CollideParams<typename PR::T> P, int nti, int ntk,
+                                                            int jchunk, float rho0) {
+    using T = typename PR::T;
+    using D = typename PR::D;
+    using S = typename PR::S;
+    static_assert(!PR::kShift, "whole populations only");
+    constexpr int NR = RI + 1, NK = KT + 1, REC = NR * NK, NT = RI * KT;
+    __shared__ S slab[2][REC * kSlabPad];
+
+    const uint32_t b = blockIdx.x;
+    const int tk = (int)(b % (uint32_t)ntk);
+    const int ti = (int)((b / (uint32_t)ntk) % (uint32_t)nti);
+    const int tj = (int)(b / ((uint32_t)ntk * (uint32_t)nti));
+    const int i0 = 1 + ti * RI, k0 = 1 + tk * KT;
+    const int ri = min(RI, g.xg1 - i0 + 1), rk = min(KT, g.zg1 - k0 + 1);  // cells of the tile
+    const int js = 1 + tj * jchunk;
+    if (js > g.yg1) return;
+    const int je = min(g.yg1, js + jchunk - 1);
+    const int tid = (int)threadIdx.x;
+    const uint32_t txm = slot_twist_mask<EVEN>(0), tym = slot_twist_mask<EVEN>(1), tzm = slot_twist_mask<EVEN>(2);
+
+    // A slab row (records k0 .. k0 + rk of row i in plane j) is RW consecutive
+    // words in HBM, except that the +1 record past the last interior cell is
+    // the wrapped k = 1.  Lane t copies words c * NT + t of every row: its word
+    // offsets, LDS positions and slot bits are fixed for the tile, and only the
+    // row's base address changes with the row and the plane.
+    constexpr int RW = NK * kQ, RC = (RW + NT - 1) / NT, ITER = NR * RC;
+    int goff[RC], loff[RC];
+    uint32_t wbits[RC];  // bit 0: word in the tile's slab, 1: twist in x, 2: in y, 3: owner k inside the tile
+    const bool kwrap = k0 + rk > g.zg1;
+#pragma unroll
+    for (int c = 0; c < RC; c++) {
+        const int wr = c * NT + tid, kk = wr / kQ, sl = wr - kk * kQ;
+        const bool ok = wr < RW && kk <= rk;
+        goff[c] = (kwrap && kk == rk) ? (1 - k0) * kQ + sl : wr;  // the wrapped record sits at k = 1
+        loff[c] = kk * kSlabPad + sl;
+        const int okk = kk - (int)((tzm >> sl) & 1u);
+        wbits[c] = (ok ? 1u : 0u) | (((txm >> sl) & 1u) << 1) | (((tym >> sl) & 1u) << 2) |
+                   ((okk >= 0 && okk < rk) ? 8u : 0u);
+    }
+    // word offset of record (i0 + r, j, k0): the row past the tile wraps to i = 1
+    auto row_base = [&](int r, int j) __attribute__((always_inline)) -> int64_t {
+        int i = i0 + r;
+        if (i > g.xg1) i -= g.xg1;
+        const int jp = j > g.yg1 ? j - g.yg1 : j;
+        return (int64_t)cell_index(g, i, jp, k0) * kQ;
+    };
+    // every load of a slab issued before any is used
+    auto fetch = [&](S (&reg)[ITER], int j) __attribute__((always_inline)) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            if (r > ri) break;
+            const S* row = Q + row_base(r, j);
+#pragma unroll
+            for (int c = 0; c < RC; c++)
+                if (wbits[c] & 1u) reg[r * RC + c] = row[goff[c]];
+        }
+    };
+    auto put = [&](const S (&reg)[ITER], int buf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            if (r > ri) break;
+#pragma unroll
+            for (int c = 0; c < RC; c++)
+                if (wbits[c] & 1u) slab[buf][r * NK * kSlabPad + loff[c]] = reg[r * RC + c];
+        }
+    };
+    // write back the words of slab buf (plane j) owned by cells of this tile in
+    // planes js .. last: owner row r - tx inside the tile, owner k inside the
+    // tile, owner plane j - ty inside js .. last
+    auto store = [&](int buf, int j, int last) __attribute__((always_inline)) {
+        const bool pl0 = j >= js && j <= last, pl1 = j - 1 >= js && j - 1 <= last;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            if (r > ri) break;
+            S* row = Q + row_base(r, j);
+#pragma unroll
+            for (int c = 0; c < RC; c++) {
+                const uint32_t wb = wbits[c];
+                const bool tx = wb & 2u, ty = wb & 4u;
+                const bool rok = r == ri ? tx : (r == 0 ? !tx : true);
+                if ((wb & 1u) && (wb & 8u) && rok && (ty ? pl1 : pl0))
+                    row[goff[c]] = slab[buf][r * NK * kSlabPad + loff[c]];
+            }
+        }
+    };
+
+    const int r = tid / KT, kk = tid - (tid / KT) * KT;
+    const bool active = r < ri && kk < rk;
+    const int ci = i0 + (active ? r : 0), ck = k0 + (active ? kk : 0);
+    // Nu and the F-occupancy word of a cell, fetched a plane ahead
+    auto cell_pre = [&](int j, T& nu, uint64_t& mw) __attribute__((always_inline)) {
+        nu = P.use_les ? load_nu<PR>(Nu, cell_index(g, ci, j, ck)) : T(0);
+        mw = fmask[mask_word_index(g, ci, j, ck)];
+    };
+    auto collide = [&](int j, int A, int B, T nuc, uint64_t mw) __attribute__((always_inline)) {
+        if (!active) return;
+        const uint32_t n = cell_index(g, ci, j, ck);
+        T v[kQ];
+#pragma unroll
+        for (int d = 0; d < kQ; d++) {
+            const int at = ((r + twist_dx(d)) * NK + kk + twist_dz(d)) * kSlabPad + (EVEN ? d : opposite(d));
+            v[d] = twist_dy(d) ? slab[B][at] : slab[A][at];
+        }
+        T fx = T(0), fy = T(0), fz = T(0);
+        if ((mw >> (ck & 63)) & 1ull) {
+            fx = F[3 * (uint64_t)n + 0];
+            fy = F[3 * (uint64_t)n + 1];
+            fz = F[3 * (uint64_t)n + 2];
+        }
+        T m[kQ], p[kQ];
+        es_moments<T, D>(v, m);
+        es_collide<T, D>(m, fx, fy, fz, nuc, P, p);
+        if (P.use_les) store_nu<PR>(Nu, n, nuc);
+#pragma unroll
+        for (int d = 0; d < kQ; d++) {
+            const int e = opposite(d);
+            const int at = ((r + twist_dx(e)) * NK + kk + twist_dz(e)) * kSlabPad + (EVEN ? e : opposite(e));
+            if (twist_dy(e)) slab[B][at] = p[d];
+            else slab[A][at] = p[d];
+        }
+    };
+    // Two slabs in LDS (planes j, j + 1) and one in flight in registers: plane
+    // j + 2 is requested while plane j collides and fills the buffer of plane j
+    // once that is written back.
+    S reg[ITER];
+    fetch(reg, js);
+    put(reg, 0);
+    fetch(reg, js + 1);
+    put(reg, 1);
+    T nu_c, nu_n = T(0);
+    uint64_t mw_c, mw_n = 0;
+    cell_pre(js, nu_c, mw_c);
+    __syncthreads();
+    int A = 0;
+    for (int j = js; j <= je; j++) {
+        const int B = A ^ 1;
+        const bool next = j + 2 <= je + 1;
+        if (next) fetch(reg, j + 2);
+        if (j + 1 <= je) cell_pre(j + 1, nu_n, mw_n);
+        collide(j, A, B, nu_c, mw_c);
+        __syncthreads();
+        store(A, j, je);
+        if (next) put(reg, A);
+        __syncthreads();
+        A = B;
+        nu_c = nu_n;
+        mw_c = mw_n;
+    }
+    // plane je + 1: the words of the chunk's last plane (twist in y)
+    store(A, je + 1, je);
 }
 
 // ------------------------------------------------------------------ Simple
@@ -1772,6 +1960,17 @@ inline void row_launch(const Grid& g, uint32_t& nkb, uint32_t& nblocks, uint32_t
     nblocks = nkb * (uint32_t)g.yg1 * (uint32_t)g.xg1;
 }
 
+// the LDS-staged IJKL step: tile RI x KT cells, about kIjklBlocks workgroups
+constexpr int kIjklTileI = 8, kIjklTileK = 32, kIjklBlocks = 6144;
+// TDLB_IJKL_LDS=0 selects the one-row kernel for IJKL (for comparison)
+inline bool ijkl_lds() {
+    static const bool on = [] {
+        const char* v = std::getenv("TDLB_IJKL_LDS");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
 template <typename PR, int L>
 hipError_t esotwist_t(int mode, int even, void* Q, void* mout, const void* F, void* Nu, const uint64_t* fm,
                       const Grid& g, const CollideArgs& c, float rho0, hipStream_t s) {
@@ -1792,6 +1991,25 @@ hipError_t esotwist_t(int mode, int even, void* Q, void* mout, const void* F, vo
         if constexpr (L == LayoutLIJK) {
             if (line_rows<S>(g)) {
                 if (even) TDLB_ESO(MODE_FUSED, 1, true); else TDLB_ESO(MODE_FUSED, 0, true);
+                return hipGetLastError();
+            }
+        }
+        if constexpr (L == LayoutIJKL && !PR::kShift && std::is_same<typename PR::T, float>::value) {
+            if (ijkl_lds()) {
+                constexpr int RI = kIjklTileI, KT = kIjklTileK;
+                const int nti = (g.xg1 + RI - 1) / RI, ntk = (g.zg1 + KT - 1) / KT;
+                // j chunks: enough workgroups to fill the GPU several times over
+                const int cols = nti * ntk;
+                const int want = (kIjklBlocks + cols - 1) / cols;
+                const int jchunk = std::max(4, (g.yg1 + want - 1) / want);
+                const int nch = (g.yg1 + jchunk - 1) / jchunk;
+                const dim3 grid((uint32_t)(cols * nch)), block((uint32_t)(RI * KT));
+                if (even)
+                    hipLaunchKernelGGL((k_esotwist_ijkl_lds<PR, 1, RI, KT>), grid, block, 0, s, q, f, nu, fm, g, P, nti,
+                                       ntk, jchunk, rho0);
+                else
+                    hipLaunchKernelGGL((k_esotwist_ijkl_lds<PR, 0, RI, KT>), grid, block, 0, s, q, f, nu, fm, g, P, nti,
+                                       ntk, jchunk, rho0);
                 return hipGetLastError();
             }
         }
