@@ -1252,11 +1252,7 @@ int tdlb_cu_bind_rushton(tdlb_cu_t cu, const tdlb_rushton_params* p, uint64_t st
     Rushton R(*p);
     const BladeEnum be = R.blade_enum();
     if (p->n_blades < 1 || p->n_blades > 16) return fail(TDLB_ERR_INVALID, "n_blades must be 1..16");
-    // Blade colours are pairwise non-overlapping when blades of one colour are
-    // more than 3*sqrt(2) apart at the inner radius.
-    const double sep = 2.0 * p->blade_inner_radius * std::sin(3.14159265358979323846 * be.n_bcol / p->n_blades);
-    if (be.n_bcol * 1 > p->n_blades || sep < 3.0 * std::sqrt(2.0) + 1e-9)
-        return fail(TDLB_ERR_UNSUPPORTED, "tank too small for on-device blade colouring (use tdlb_cu_forcing)");
+    // blade colour classes never overlap (blade_colours: a colour per blade in small tanks)
     // blades are generated on the device at any angle: their whole swept disk and
     // the planes they occupy must lie inside the interior
     {
@@ -1293,6 +1289,7 @@ int tdlb_cu_bind_rushton(tdlb_cu_t cu, const tdlb_rushton_params* p, uint64_t st
         return fail(TDLB_ERR_NOMEM, "blade moment cache");
 
     cu->blade.n_blades = p->n_blades;
+    cu->blade.n_bcol = be.n_bcol;
     cu->blade.j_lo = (int32_t)p->blade_plane_lo;
     cu->blade.n_planes = be.n_planes;
     cu->blade.n_radial = be.n_radial;

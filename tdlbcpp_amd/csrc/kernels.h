@@ -47,6 +47,7 @@ struct PointsDev {
 // Blade layout for on-device generation of the rotating (updating) IB points.
 struct BladeLayout {
     int32_t n_blades;
+    int32_t n_bcol;          // blade colours (blade_colours of the host generator)
     int32_t j_lo, n_planes;  // first plane (pre-ghost j) and number of planes
     int32_t n_radial;        // points per blade per plane
     double r_inner;          // radius of the first point; spacing 1

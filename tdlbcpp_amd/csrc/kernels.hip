@@ -630,6 +630,172 @@ __global__ __launch_bounds__(kRowBlock) void k_simple(const typename PR::S* src,
     simple_cell<PR, L, MODE>(src, dst, mout, F, Nu, fmask, g, P, rho0, rb.i, rb.j, k);
 }
 
+// ------------------------------------------------------------------ Simple pull, IJKL through LDS
+// index of direction d among the five with e_y = +1
+TDLB_HD constexpr int ypos_index(int d) {
+    int n = 0;
+    for (int e = 0; e < d; e++)
+        if (dir_y(e) > 0) n++;
+    return n;
+}
+
+// The two-lattice pull in IJKL (src -> dst), staged like k_esotwist_ijkl_lds:
+// a workgroup owns a tile of RI x KT cells in (i, k) and marches along j with
+// two slabs of src records in LDS, rows i0 - 1 .. i0 + RI and cells
+// k0 - 1 .. k0 + KT (the +-1 halo; the ghost layer is real memory and the pull
+// never uses it) of planes j and j + 1.  The five directions with e_y = +1 pull
+// from plane j - 1: a lane reads them for its next cell while plane j - 1 is
+// still in LDS and carries them in registers.  Each cell writes its whole
+// record of dst, so no word of dst has two writers and the slabs are read-only.
+template <typename PR, int RI, int KT>
+__global__ __launch_bounds__(RI * KT) void k_simple_ijkl_lds(const typename PR::S* __restrict__ src,
+                                                          typename PR::S* __restrict__ dst,
+                                                          const typename PR::T* __restrict__ F,
+                                                          typename PR::N* __restrict__ Nu,
+                                                          const uint64_t* __restrict__ fmask, Grid g,
+                                                          CollideParams<typename PR::T> P, int nti, int ntk,
+                                                          int jchunk, float rho0) {
+    using T = typename PR::T;
+    using D = typename PR::D;
+    using S = typename PR::S;
+    static_assert(!PR::kShift, "whole populations only");
+    constexpr int NR = RI + 2, NK = KT + 2, REC = NR * NK, NT = RI * KT;
+    __shared__ S slab[2][REC * kSlabPad];
+
+    const uint32_t b = blockIdx.x;
+    const int tk = (int)(b % (uint32_t)ntk);
+    const int ti = (int)((b / (uint32_t)ntk) % (uint32_t)nti);
+    const int tj = (int)(b / ((uint32_t)ntk * (uint32_t)nti));
+    const int i0 = 1 + ti * RI, k0 = 1 + tk * KT;
+    const int ri = min(RI, g.xg1 - i0 + 1), rk = min(KT, g.zg1 - k0 + 1);
+    const int js = 1 + tj * jchunk;
+    if (js > g.yg1) return;
+    const int je = min(g.yg1, js + jchunk - 1);
+    const int tid = (int)threadIdx.x;
+
+    // slab row r holds records k0 - 1 .. k0 + rk of row i0 - 1 + r: RW words
+    // from the word offset of record (i0 - 1 + r, j, k0 - 1)
+    constexpr int RW = NK * kQ, RC = (RW + NT - 1) / NT, ITER = NR * RC;
+    int loff[RC];
+    bool wok[RC];
+#pragma unroll
+    for (int c = 0; c < RC; c++) {
+        const int wr = c * NT + tid, kk = wr / kQ, sl = wr - kk * kQ;
+        wok[c] = wr < RW && kk <= rk + 1;
+        loff[c] = kk * kSlabPad + sl;
+    }
+    auto row_base = [&](int r, int j) __attribute__((always_inline)) -> int64_t {
+        return (int64_t)cell_index(g, i0 - 1 + r, j, k0 - 1) * kQ;
+    };
+    auto fetch = [&](S (&reg)[ITER], int j) __attribute__((always_inline)) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            if (r > ri + 1) break;
+            const S* row = src + row_base(r, j);
+#pragma unroll
+            for (int c = 0; c < RC; c++) {
+                int t = c * NT + tid;
+                asm volatile("" : "+v"(t));  // not hoisted across the plane loop
+                if (wok[c]) reg[r * RC + c] = row[t];
+            }
+        }
+    };
+    auto put = [&](const S (&reg)[ITER], int buf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            if (r > ri + 1) break;
+#pragma unroll
+            for (int c = 0; c < RC; c++)
+                if (wok[c]) slab[buf][r * NK * kSlabPad + loff[c]] = reg[r * RC + c];
+        }
+    };
+
+    const int r = tid / KT, kk = tid - (tid / KT) * KT;
+    const bool active = r < ri && kk < rk;
+    const int ci = i0 + (active ? r : 0), ck = k0 + (active ? kk : 0);
+    const bool im = ci >= 2, ip = ci <= g.xg1 - 1, km = ck >= 2, kp = ck <= g.zg1 - 1;
+    // the source c - e_d of direction d is interior in x and z
+    auto xz_ok = [&](int d) __attribute__((always_inline)) {
+        const int ex = dir_x(d), ez = dir_z(d);
+        return (ex > 0 ? im : ex < 0 ? ip : true) && (ez > 0 ? km : ez < 0 ? kp : true);
+    };
+    // slab position of slot d of the source (i - e_x, k - e_z), or of the cell itself
+    auto src_at = [&](int d, bool shift) __attribute__((always_inline)) -> int {
+        const int dr = shift ? -dir_x(d) : 0, dk = shift ? -dir_z(d) : 0;
+        return ((r + 1 + dr) * NK + kk + 1 + dk) * kSlabPad + d;
+    };
+    // the e_y = +1 values of the next plane's cell (its sources lie in plane buf),
+    // used only where the source is interior
+    constexpr int NY = 5;
+    auto carry_from = [&](int buf, T (&cy)[NY]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int d = 0; d < kQ; d++)
+            if (dir_y(d) > 0) cy[ypos_index(d)] = slab[buf][src_at(d, true)];
+    };
+    auto cell_pre = [&](int j, T& nu, uint64_t& mw) __attribute__((always_inline)) {
+        nu = P.use_les ? load_nu<PR>(Nu, cell_index(g, ci, j, ck)) : T(0);
+        mw = fmask[mask_word_index(g, ci, j, ck)];
+    };
+
+    S reg[ITER];
+    fetch(reg, js - 1);
+    put(reg, 0);
+    fetch(reg, js);
+    put(reg, 1);
+    __syncthreads();
+    T cy[NY];
+    carry_from(0, cy);
+    if (js + 1 <= g.yg1 + 1) fetch(reg, js + 1);
+    __syncthreads();
+    put(reg, 0);
+    T nu_c, nu_n = T(0);
+    uint64_t mw_c, mw_n = 0;
+    cell_pre(js, nu_c, mw_c);
+    __syncthreads();
+    int A = 1;  // plane j; A ^ 1 holds plane j + 1
+    for (int j = js; j <= je; j++) {
+        const int B = A ^ 1;
+        const bool next = j + 2 <= je + 1;
+        if (next) fetch(reg, j + 2);
+        if (j + 1 <= je) cell_pre(j + 1, nu_n, mw_n);
+        if (active) {
+            const bool jm = j >= 2, jp = j <= g.yg1 - 1;
+            T v[kQ];
+#pragma unroll
+            for (int d = 0; d < kQ; d++) {
+                // the pull of simple_cell: the source when it is interior in every axis, else the cell itself
+                const int ey = dir_y(d);
+                const bool ok = xz_ok(d) && (ey > 0 ? jm : ey < 0 ? jp : true);
+                if (!ok) v[d] = slab[A][src_at(d, false)];
+                else if (ey > 0) v[d] = cy[ypos_index(d)];
+                else if (ey < 0) v[d] = slab[B][src_at(d, true)];
+                else v[d] = slab[A][src_at(d, true)];
+            }
+            carry_from(A, cy);  // v holds this cell's values; cy now the next cell's
+            const uint32_t cn = cell_index(g, ci, j, ck);
+            T fx = T(0), fy = T(0), fz = T(0);
+            if ((mw_c >> (ck & 63)) & 1ull) {
+                fx = F[3 * (uint64_t)cn + 0];
+                fy = F[3 * (uint64_t)cn + 1];
+                fz = F[3 * (uint64_t)cn + 2];
+            }
+            T m[kQ], p[kQ];
+            es_moments<T, D>(v, m);
+            es_collide<T, D>(m, fx, fy, fz, nu_c, P, p);
+            if (P.use_les) store_nu<PR>(Nu, cn, nu_c);
+            S* out = dst + (uint64_t)cn * kQ;
+#pragma unroll
+            for (int d = 0; d < kQ; d++) out[d] = p[d];
+        }
+        __syncthreads();
+        if (next) put(reg, A);  // plane j + 2 takes the buffer of plane j
+        __syncthreads();
+        A = B;
+        nu_c = nu_n;
+        mw_c = mw_n;
+    }
+}
+
 // ------------------------------------------------------------------ data movement on every cell
 
 // push: dst[c][d] = src[c-e_d][d] if c-e_d is interior, else src[c][d] (all cells)
@@ -1260,7 +1426,7 @@ __device__ __forceinline__ BladeEnum blade_enum_of(const BladeLayout& bl) {
     e.n_blades = bl.n_blades;
     e.n_planes = bl.n_planes;
     e.n_radial = bl.n_radial;
-    e.n_bcol = (bl.n_blades % 2 == 0) ? 2 : 3;
+    e.n_bcol = bl.n_bcol;
     return e;
 }
 __device__ __forceinline__ BladeClass blade_class(const BladeEnum& e, int c, int64_t base) {
@@ -1901,7 +2067,7 @@ __device__ __forceinline__ void blade_gen_point(const BladeLayout& bl, const Bla
     e.n_blades = bl.n_blades;
     e.n_planes = bl.n_planes;
     e.n_radial = bl.n_radial;
-    e.n_bcol = (bl.n_blades % 2 == 0) ? 2 : 3;
+    e.n_bcol = bl.n_bcol;
     int b, pl, r, cls;
     blade_decode(e, t, b, pl, r, cls);
     tdlb_ib_point pt;
@@ -1960,8 +2126,12 @@ inline void row_launch(const Grid& g, uint32_t& nkb, uint32_t& nblocks, uint32_t
     nblocks = nkb * (uint32_t)g.yg1 * (uint32_t)g.xg1;
 }
 
-// the LDS-staged IJKL step: tile RI x KT cells, about kIjklBlocks workgroups
-constexpr int kIjklTileI = 8, kIjklTileK = 32, kIjklBlocks = 6144;
+// the LDS-staged IJKL step: tile RI x KT cells, about kIjklBlocks workgroups.
+// Measured at 512^3 (FP32): 4 x 64 and 16384 workgroups 5.48 ms, 8 x 32 5.51 ms
+// (6144 workgroups 5.66 ms), 16 x 16 7.03 ms, 8 x 16 6.24 ms.
+constexpr int kIjklTileI = 4, kIjklTileK = 64, kIjklBlocks = 16384;
+// the Simple pull through LDS: tile RI x KT cells (halo on both sides)
+constexpr int kIjklPullI = 8, kIjklPullK = 32;
 // TDLB_IJKL_LDS=0 selects the one-row kernel for IJKL (for comparison)
 inline bool ijkl_lds() {
     static const bool on = [] {
@@ -2033,6 +2203,19 @@ hipError_t simple_t(int mode, const void* src, void* dst, void* mout, const void
     const T* f = static_cast<const T*>(F);
     auto* nu = static_cast<typename PR::N*>(Nu);
     const auto P = to_params<T, typename PR::D>(c);
+    if constexpr (L == LayoutIJKL && !PR::kShift && std::is_same<typename PR::T, float>::value) {
+        if (mode == MODE_FUSED && ijkl_lds()) {
+            constexpr int RI = kIjklPullI, KT = kIjklPullK;
+            const int nti = (g.xg1 + RI - 1) / RI, ntk = (g.zg1 + KT - 1) / KT;
+            const int cols = nti * ntk;
+            const int want = (kIjklBlocks + cols - 1) / cols;
+            const int jchunk = std::max(4, (g.yg1 + want - 1) / want);
+            const int nch = (g.yg1 + jchunk - 1) / jchunk;
+            hipLaunchKernelGGL((k_simple_ijkl_lds<PR, RI, KT>), dim3((uint32_t)(cols * nch)), dim3((uint32_t)(RI * KT)),
+                               0, s, a, b, f, nu, fm, g, P, nti, ntk, jchunk, rho0);
+            return hipGetLastError();
+        }
+    }
 #define TDLB_SIM(M) \
     hipLaunchKernelGGL((k_simple<PR, L, M>), dim3(nb), dim3(bk), 0, s, a, b, mo, f, nu, fm, g, P, nkb, rho0)
     switch (mode) {

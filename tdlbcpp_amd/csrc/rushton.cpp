@@ -84,7 +84,7 @@ BladeEnum Rushton::blade_enum() const {
     e.n_blades = p.n_blades;
     e.n_planes = (int32_t)p.blade_planes;
     e.n_radial = (int32_t)std::floor(p.blade_outer_radius - p.blade_inner_radius + 1e-9) + 1;
-    e.n_bcol = (p.n_blades % 2 == 0) ? 2 : 3;
+    e.n_bcol = blade_colours(p.n_blades, p.blade_inner_radius);
     return e;
 }
 

@@ -45,8 +45,20 @@ namespace tdlb {
 
 struct BladeEnum {
     int32_t n_blades, n_planes, n_radial;
-    int32_t n_bcol;  // blade colours: 2 for an even blade count, else 3
+    int32_t n_bcol;  // blade colours (blade_colours)
 };
+
+// Blade colours: blades b, b + n_bcol, ... share a colour class, so their
+// stencils (3 x 3 cells in (i, k)) must not overlap: 2 colours for an even
+// blade count, 3 for an odd one, when blades of one colour are more than
+// 3 sqrt(2) cells apart at the inner radius; otherwise (small tanks) every
+// blade is a colour of its own.
+TDLB_HD inline int32_t blade_colours(int32_t n_blades, double r_inner) {
+    const int32_t c = (n_blades % 2 == 0) ? 2 : 3;
+    if (c >= n_blades) return n_blades > 0 ? n_blades : 1;
+    const double sep = 2.0 * r_inner * sin(3.14159265358979323846 * c / n_blades);
+    return sep < 3.0 * 1.4142135623730951 + 1e-9 ? n_blades : c;
+}
 
 // Number of blade points of colour class c (c = bcol*5 + rmod).
 TDLB_HD inline int64_t blade_class_size(const BladeEnum& e, int c) {

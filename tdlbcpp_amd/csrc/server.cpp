@@ -236,8 +236,8 @@ int tdlb_server_destroy(tdlb_server_t s) {
 
 int tdlb_server_submit(tdlb_server_t s, const tdlb_tank_request* req, int64_t* ticket) {
     if (!s || !req) return fail(TDLB_ERR_INVALID, "null argument");
-    if (req->edge < 36 || req->steps < 1 || req->tip_speed < 0 || req->reynolds <= 0)
-        return fail(TDLB_ERR_INVALID, "request: edge >= 36, steps >= 1, tip_speed >= 0, reynolds > 0");
+    if (req->edge < 12 || req->steps < 1 || req->tip_speed < 0 || req->reynolds <= 0)
+        return fail(TDLB_ERR_INVALID, "request: edge >= 12, steps >= 1, tip_speed >= 0, reynolds > 0");
     auto j = std::make_unique<Job>();
     j->ticket = s->next_ticket++;
     j->req = *req;

@@ -147,7 +147,7 @@ def test_gpu_forcing_bit_exact_with_overlaps_and_wrap(layout, stream):
 @pytest.mark.parametrize("layout", LAYOUTS, ids=LAYOUT_IDS)
 @pytest.mark.parametrize("stream", STREAMS, ids=["Simple", "Esotwist"])
 def test_gpu_main_loop_20_steps_matches_oracle(layout, stream):
-    T = 36  # smallest tank above 30 whose blades pass the on-device colouring check
+    T = 36
     case(f"main loop T={T} 20 steps {layout} {stream}")
     oflow, gflow = flow_for_tank(T)
     o = O.OracleCU(T, dtype=O.F32, layout=layout[1], streaming=stream[1], use_les=True, **oflow)
@@ -167,6 +167,29 @@ def test_gpu_main_loop_20_steps_matches_oracle(layout, stream):
     exact = np.array_equal(g.get(td.FIELD_Q), o.get(O.FIELD_Q))
     print(f"[gpu case] bit-identical Q after 20 steps: {exact}", flush=True)
     assert exact
+
+
+@pytest.mark.parametrize("T", [12, 20, 31, 35])
+def test_gpu_small_tanks_match_oracle(T):
+    """Tanks below 36, where blades of one colour class would come within 3 sqrt(2)
+    cells of each other at the inner radius, give every blade a colour class of
+    its own: they bind and run the fused loop bit-identical to the oracle."""
+    case(f"small tank T={T} Esotwist/LIJK and Simple/IJKL")
+    for layout, stream in ((LAYOUTS[1], STREAMS[1]), (LAYOUTS[0], STREAMS[0])):
+        oflow, gflow = flow_for_tank(T)
+        o = O.OracleCU(T, dtype=O.F32, layout=layout[1], streaming=stream[1], use_les=True, **oflow)
+        g = td.ComputeUnit(T, layout=layout[0], streaming=stream[0], precision=td.PREC_F32, **gflow)
+        rp = td.rushton_defaults(T)
+        o.initialise(1.0)
+        g.initialise(1.0)
+        ref = ReferenceLoop(o, rp)
+        g.bind_rushton(rp, step=1, angle=0.0)
+        ref.run(8)
+        g.run(8)
+        np.testing.assert_array_equal(g.rotating_points()["i"], ref.rot["i"])
+        np.testing.assert_array_equal(g.get(td.FIELD_F), o.get(O.FIELD_F))
+        np.testing.assert_array_equal(g.get(td.FIELD_Q), o.get(O.FIELD_Q))
+        g.close()
 
 
 @pytest.mark.parametrize("T,steps,team_cost", [(64, 20, None), (64, 10, "8"), (128, 6, None)],

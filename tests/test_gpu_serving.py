@@ -39,7 +39,7 @@ def drain(s, limit=10000):
 def test_served_tanks_equal_solo_runs(cfg):
     prec, st, ly = cfg
     case(f"serving vs solo {cfg}")
-    reqs = [(36, 0.10, 5000.0, 30), (40, 0.07, 12000.0, 20), (48, 0.12, 3000.0, 9), (36, 0.05, 20000.0, 16)]
+    reqs = [(36, 0.10, 5000.0, 30), (40, 0.07, 12000.0, 20), (48, 0.12, 3000.0, 9), (20, 0.05, 20000.0, 16)]
     s = td.Server(prec, ly, st, tick_steps=7, keep_handles=True)
     tickets = [s.submit(*reqs[0])]
     results = []
