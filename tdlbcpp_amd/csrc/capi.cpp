@@ -726,8 +726,8 @@ int tdlb_probe_planes(int32_t device_id, int32_t elem_bytes, int64_t cells, int6
     // default stride: the plane rounded up to 1 MiB
     const uint64_t plane_bytes = plane_stride_bytes > 0 ? (uint64_t)plane_stride_bytes
                                                         : ((uint64_t)cells * elem_bytes + align - 1) / align * align;
-    if (plane_bytes < (uint64_t)cells * elem_bytes || plane_bytes % 256 != 0)
-        return fail(TDLB_ERR_INVALID, "plane_stride_bytes: at least cells * elem_bytes, a multiple of 256");
+    if (plane_bytes < (uint64_t)cells * elem_bytes || plane_bytes % 128 != 0)
+        return fail(TDLB_ERR_INVALID, "plane_stride_bytes: at least cells * elem_bytes, a multiple of 128");
     HIPCHK(hipSetDevice(device_id < 0 ? 0 : device_id));
     void* base = nullptr;
     if (hipMalloc(&base, plane_bytes * planes) != hipSuccess) return fail(TDLB_ERR_NOMEM, "probe planes");

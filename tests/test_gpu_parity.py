@@ -169,6 +169,29 @@ def test_gpu_main_loop_20_steps_matches_oracle(layout, stream):
     assert exact
 
 
+@pytest.mark.parametrize("layout,stream", [(LAYOUTS[0], STREAMS[1]), (LAYOUTS[0], STREAMS[0]),
+                                           (LAYOUTS[1], STREAMS[1])], ids=["IJKL-Esotwist", "IJKL-Simple",
+                                                                           "LIJK-Esotwist"])
+def test_gpu_non_cubic_grid_matches_oracle(layout, stream):
+    """A tank of diameter 42 and height 29 in a 42 x 29 x 53 grid: the IJKL
+    kernels' tiles are partial in i and k (and wrap in k), the last chunk of
+    planes holds one plane; 6 main-loop steps bit-identical to the oracle."""
+    X, Y, Z = 42, 29, 53
+    case(f"non-cubic {X}x{Y}x{Z} {layout} {stream}")
+    oflow, gflow = flow_for_tank(X)
+    o = O.OracleCU(X, Y, Z, dtype=O.F32, layout=layout[1], streaming=stream[1], use_les=True, **oflow)
+    g = td.ComputeUnit(X, Y, Z, layout=layout[0], streaming=stream[0], precision=td.PREC_F32, **gflow)
+    rp = td.rushton_defaults(X, height=Y)
+    o.initialise(1.0)
+    g.initialise(1.0)
+    ref = ReferenceLoop(o, rp)
+    g.bind_rushton(rp, step=1, angle=0.0)
+    ref.run(6)
+    g.run(6)
+    np.testing.assert_array_equal(g.get(td.FIELD_NU), o.get(O.FIELD_NU))
+    np.testing.assert_array_equal(g.get(td.FIELD_Q), o.get(O.FIELD_Q))
+
+
 @pytest.mark.parametrize("T", [12, 20, 31, 35])
 def test_gpu_small_tanks_match_oracle(T):
     """Tanks below 36, where blades of one colour class would come within 3 sqrt(2)
