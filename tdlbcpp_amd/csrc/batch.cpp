@@ -89,6 +89,7 @@ struct tdlb_batch {
     DevBuf<int64_t> d_tab;
     DevBuf<PlaneTask> d_tasks;
     DevBuf<BladeStep> d_steps;
+    DevBuf<int32_t> d_bm;  // member of each fused-kernel block
 
     // plan of the last run and its graph
     BatchPlan plan{};
@@ -336,6 +337,9 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     std::vector<PlaneTask> tasks(team_tasks);
     for (const auto& t : tk) tasks.push_back(t.second);
     const int64_t tot_coll = off_coll[M], tot_rot = off_rot[M], tot_words = off_words[M];
+    std::vector<int32_t> blk_member((size_t)tot_coll);
+    for (int r = 0; r < M; r++)
+        std::fill(blk_member.begin() + off_coll[r], blk_member.begin() + off_coll[r + 1], (int32_t)r);
 
     int moved = 0;
     int g;
@@ -346,6 +350,11 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     if ((g = b->d_tasks.grow(std::max<size_t>(tasks.size(), 1))) < 0) return fail(TDLB_ERR_NOMEM, "batch tasks");
     moved |= g;
     if (b->d_steps.grow(std::max<size_t>(steps.size(), 1)) < 0) return fail(TDLB_ERR_NOMEM, "batch blade table");
+    if ((g = b->d_bm.grow(std::max<size_t>(blk_member.size(), 1))) < 0) return fail(TDLB_ERR_NOMEM, "batch block table");
+    moved |= g;
+    if (!blk_member.empty())
+        HIPCHK(hipMemcpyAsync(b->d_bm.p, blk_member.data(), blk_member.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                              b->stream));
     for (int r = 0; r < M; r++) mem[r].steps = b->d_steps.p + step_base[r];
     if (M > 0) HIPCHK(hipMemcpyAsync(b->d_mem.p, mem.data(), M * sizeof(MemberDev), hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipMemcpyAsync(b->d_tab.p, tab.data(), tab.size() * sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
@@ -362,6 +371,7 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     p.mem_rw = b->d_mem.p;
     p.n_members = M;
     p.off_coll = b->d_tab.p;
+    p.coll_member = b->d_bm.p;
     p.off_rot = b->d_tab.p + (M + 1);
     p.off_words = b->d_tab.p + 2 * (M + 1);
     p.tasks = b->d_tasks.p;
@@ -377,7 +387,7 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     // handle's address can come back for a tank of another size).
     std::vector<int64_t> sig = {M, tot_coll, tot_rot, tot_words, p.n_tasks, p.bs,
                                 (int64_t)(uintptr_t)b->d_mem.p, (int64_t)(uintptr_t)b->d_tab.p,
-                                (int64_t)(uintptr_t)b->d_tasks.p};
+                                (int64_t)(uintptr_t)b->d_tasks.p, (int64_t)(uintptr_t)b->d_bm.p};
     *structure_changed = moved || sig != b->launch_sig || key != b->key || b->buffers_moved;
     b->buffers_moved = false;
     b->key = key;
@@ -451,6 +461,7 @@ int tdlb_batch_destroy(tdlb_batch_t b) {
     b->d_tab.release();
     b->d_tasks.release();
     b->d_steps.release();
+    b->d_bm.release();
     (void)hipStreamDestroy(b->stream);
     delete b;
     return TDLB_OK;

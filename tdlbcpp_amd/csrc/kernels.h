@@ -298,6 +298,7 @@ struct BatchPlan {
     MemberDev* mem_rw;
     int32_t n_members;
     const int64_t* off_coll;   // blocks of the fused kernel
+    const int32_t* coll_member;  // member of each block of the fused kernel (or null: search off_coll)
     const int64_t* off_rot;    // blade points
     const int64_t* off_words;  // occupancy-mask words
     const PlaneTask* tasks;

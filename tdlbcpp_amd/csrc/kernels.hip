@@ -2902,12 +2902,13 @@ __device__ __forceinline__ bool batch_cell(const Grid& g, int tile, uint32_t b, 
 
 template <typename PR, int L, int STREAM>
 __global__ __launch_bounds__(kRowBlock) void k_batch_fused(const MemberDev* __restrict__ mem,
-                                                        const int64_t* __restrict__ off, int n) {
+                                                        const int64_t* __restrict__ off, int n,
+                                                        const int32_t* __restrict__ bm) {
     static_assert(std::is_same<typename PR::T, float>::value, "batches compute in float");
     using S = typename PR::S;
     const uint32_t blk = layout_block<L>();
-    if ((int64_t)blk >= off[n]) return;
-    const int m = find_slot(off, n, blk);
+    // one load from the block table instead of a binary search of dependent loads
+    const int m = bm ? bm[blk] : find_slot(off, n, blk);
     const MemberDev& md = mem[m];
     if (!md.active) return;
     int i, j, k;
@@ -2994,7 +2995,7 @@ template <typename PR, int L, int STREAM>
 hipError_t batch_fused_t(const BatchPlan& p, hipStream_t s) {
     if (p.tot_coll <= 0) return hipSuccess;
     hipLaunchKernelGGL((k_batch_fused<PR, L, STREAM>), dim3((uint32_t)p.tot_coll), dim3(p.bs), 0, s, p.mem,
-                       p.off_coll, p.n_members);
+                       p.off_coll, p.n_members, p.coll_member);
     return hipGetLastError();
 }
 
