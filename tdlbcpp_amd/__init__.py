@@ -135,7 +135,7 @@ def lib():
             "tdlb_batch_member": ([vp, vp, P(u64), P(d), P(i64)], i32),
             "tdlb_batch_info": ([vp, P(i64), P(i64), P(ctypes.c_int32), i64], i32),
             "tdlb_batch_tune": ([vp, vp], i32),
-            "tdlb_batch_shape": ([vp, P(ctypes.c_int32), P(ctypes.c_int32)], i32),
+            "tdlb_batch_shape": ([vp, P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int32)], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -496,10 +496,10 @@ class Batch:
         n, l = ctypes.c_int64(), ctypes.c_int64()
         tiles = (ctypes.c_int32 * max(1, len(self.members)))()
         check(lib().tdlb_batch_info(self._b, ctypes.byref(n), ctypes.byref(l), tiles, len(self.members)))
-        bs, gs = ctypes.c_int32(), ctypes.c_int32()
-        check(lib().tdlb_batch_shape(self._b, ctypes.byref(bs), ctypes.byref(gs)))
+        bs, gs, fb = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        check(lib().tdlb_batch_shape(self._b, ctypes.byref(bs), ctypes.byref(gs), ctypes.byref(fb)))
         return {"members": n.value, "launches_per_step": l.value, "tiles": list(tiles)[:len(self.members)],
-                "block_size": bs.value, "graph_steps": gs.value}
+                "block_size": bs.value, "graph_steps": gs.value, "forcing_block": fb.value}
 
     def close(self):
         if getattr(self, "_b", None) and self._b.value:

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -375,6 +376,22 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     p.off_rot = b->d_tab.p + (M + 1);
     p.off_words = b->d_tab.p + 2 * (M + 1);
     p.tasks = b->d_tasks.p;
+    // forcing workgroup: a lane per ~4 static stencil cells of the mean task,
+    // a power of two from 128 to 1024 lanes (64 x 48^3 tanks: 128 lanes 1.6x
+    // faster than 1024; the mixed 48..160 batch: 256 lanes 4 % faster; 160^3:
+    // 1024 lanes 1 % faster)
+    {
+        double cells = 0.0;
+        for (int r = 0; r < M; r++) cells += (double)b->members[idx[r]].cu->static_set.n_cells;
+        const double mean = tasks.empty() ? 0.0 : cells / (double)tasks.size();
+        int t = 128;
+        while (t < kForcePlaneBlock && 2.0 * t <= mean / 4.0) t *= 2;
+        p.force_bs = t;
+    }
+    if (const char* v = std::getenv("TDLB_BATCH_FORCE_BLOCK")) {
+        const int t = std::atoi(v);
+        if (t >= 64 && t <= kForcePlaneBlock && t % 64 == 0) p.force_bs = t;
+    }
     p.tot_coll = tot_coll;
     p.tot_rot = tot_rot;
     p.tot_words = tot_words;
@@ -385,7 +402,7 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     // A captured step bakes in every launch size and kernel argument: recapture
     // whenever any of them differs (member identity alone is not enough, a freed
     // handle's address can come back for a tank of another size).
-    std::vector<int64_t> sig = {M, tot_coll, tot_rot, tot_words, p.n_tasks, p.bs,
+    std::vector<int64_t> sig = {M, tot_coll, tot_rot, tot_words, p.n_tasks, p.bs, p.force_bs,
                                 (int64_t)(uintptr_t)b->d_mem.p, (int64_t)(uintptr_t)b->d_tab.p,
                                 (int64_t)(uintptr_t)b->d_tasks.p, (int64_t)(uintptr_t)b->d_bm.p};
     *structure_changed = moved || sig != b->launch_sig || key != b->key || b->buffers_moved;
@@ -507,10 +524,11 @@ int tdlb_batch_tune(tdlb_batch_t b, tdlb_cu_t cu) {
     return tune_shape(b, cu, true, &tt, cu->stream);  // the arrays it wrote are reset by tdlb_cu_initialise
 }
 
-int tdlb_batch_shape(tdlb_batch_t b, int32_t* block_size, int32_t* graph_steps) {
+int tdlb_batch_shape(tdlb_batch_t b, int32_t* block_size, int32_t* graph_steps, int32_t* forcing_block) {
     if (!b) return fail(TDLB_ERR_INVALID, "null batch");
     if (block_size) *block_size = b->bs;
     if (graph_steps) *graph_steps = b->use_graph ? kGraphSteps : 1;
+    if (forcing_block) *forcing_block = b->plan.force_bs;
     return TDLB_OK;
 }
 

@@ -157,6 +157,7 @@ struct ForceGeom {
 constexpr int kBarWords = 4;
 // points one pass of a plane task handles: 16 wavefronts of 7 groups of 9 lanes
 constexpr int kForcePlanePoints = 112;
+constexpr int kForcePlaneBlock = 1024;  // threads of a plane task's workgroup
 // One forcing pass of a bound tank (every plane of fg): next must already hold
 // the static occupancy bits; blade stencil cells missing from prev get F = 0
 // and their bit in next.
@@ -304,6 +305,7 @@ struct BatchPlan {
     const PlaneTask* tasks;
     int64_t tot_coll, tot_rot, tot_words, n_tasks;
     int32_t bs;  // threads per block of the fused kernel (256 or 512)
+    int32_t force_bs;  // threads per workgroup of the forcing pass (64 .. kForcePlaneBlock, a multiple of 64)
 };
 
 // Enqueue `steps` main-loop steps of every active member: per step blade

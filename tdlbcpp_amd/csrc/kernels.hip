@@ -1350,7 +1350,7 @@ __global__ __launch_bounds__(kBlock) void k_force_level_cached(PointsDev p, int6
 // points runs on a team of workgroups that meet at a barrier in global memory
 // (team_barrier); teams are the first workgroups of the launch, so they are
 // dispatched, and resident, before any other workgroup of it.
-constexpr int kPlaneBlock = 1024;
+constexpr int kPlaneBlock = kForcePlaneBlock;
 constexpr int kPlanePoints = (kPlaneBlock / 64) * kGroupsPerWave;
 static_assert(kPlanePoints == kForcePlanePoints, "points per pass of a plane task");
 // polls of a team barrier before it gives up and flags a time-out (about a
@@ -3012,8 +3012,8 @@ hipError_t batch_step_t(const BatchPlan& p, hipStream_t s) {
         hipLaunchKernelGGL(k_batch_mask_init, dim3(nblk(p.tot_words, kBlock)), dim3(kBlock), 0, s, p.mem,
                            p.off_words, n, p.tot_words);
     if (p.n_tasks > 0)
-        hipLaunchKernelGGL((k_batch_force_planes<PR, L, STREAM>), dim3((uint32_t)p.n_tasks), dim3(kPlaneBlock), 0, s,
-                           p.mem, p.tasks);
+        hipLaunchKernelGGL((k_batch_force_planes<PR, L, STREAM>), dim3((uint32_t)p.n_tasks),
+                           dim3(p.force_bs > 0 ? p.force_bs : kPlaneBlock), 0, s, p.mem, p.tasks);
     hipLaunchKernelGGL(k_batch_advance<STREAM>, dim3(nblk(n, 64)), dim3(64), 0, s, p.mem_rw, n);
     return hipGetLastError();
 }
