@@ -376,17 +376,10 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     p.off_rot = b->d_tab.p + (M + 1);
     p.off_words = b->d_tab.p + 2 * (M + 1);
     p.tasks = b->d_tasks.p;
-    // forcing workgroup: a lane per ~4 static stencil cells of the mean task,
-    // a power of two from 128 to 1024 lanes (64 x 48^3 tanks: 128 lanes 1.6x
-    // faster than 1024; the mixed 48..160 batch: 256 lanes 4 % faster; 160^3:
-    // 1024 lanes 1 % faster)
     {
         double cells = 0.0;
         for (int r = 0; r < M; r++) cells += (double)b->members[idx[r]].cu->static_set.n_cells;
-        const double mean = tasks.empty() ? 0.0 : cells / (double)tasks.size();
-        int t = 128;
-        while (t < kForcePlaneBlock && 2.0 * t <= mean / 4.0) t *= 2;
-        p.force_bs = t;
+        p.force_bs = forcing_block_for(tasks.empty() ? 0.0 : cells / (double)tasks.size());
     }
     if (const char* v = std::getenv("TDLB_BATCH_FORCE_BLOCK")) {
         const int t = std::atoi(v);
