@@ -101,6 +101,7 @@ struct tdlb_batch {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int64_t launches_per_step = 0;
     int32_t bs = 256;                                // fused-kernel block size of the plan
+    int32_t n_cu = 256;                              // compute units of the device
     std::vector<int> active_idx;  // members in the plan (indices into members)
 };
 
@@ -376,10 +377,13 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     p.off_rot = b->d_tab.p + (M + 1);
     p.off_words = b->d_tab.p + 2 * (M + 1);
     p.tasks = b->d_tasks.p;
-    {
+    // small forcing workgroups only with enough tasks to keep every CU busy
+    // several times over (a batch of one tank runs faster with 1024 lanes)
+    p.force_bs = kForcePlaneBlock;
+    if ((int64_t)tasks.size() >= 4 * (int64_t)b->n_cu) {
         double cells = 0.0;
         for (int r = 0; r < M; r++) cells += (double)b->members[idx[r]].cu->static_set.n_cells;
-        p.force_bs = forcing_block_for(tasks.empty() ? 0.0 : cells / (double)tasks.size());
+        p.force_bs = forcing_block_for(cells / (double)tasks.size());
     }
     if (const char* v = std::getenv("TDLB_BATCH_FORCE_BLOCK")) {
         const int t = std::atoi(v);
@@ -458,6 +462,9 @@ int tdlb_batch_create(int32_t precision, int32_t layout, int32_t streaming, int3
         delete b;
         return fail(TDLB_ERR_HIP, "batch stream");
     }
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device) == hipSuccess && ncu > 0)
+        b->n_cu = ncu;
     *out = b;
     return TDLB_OK;
 }
