@@ -155,8 +155,8 @@ struct ForceGeom {
                              // (launches deal workgroups to the XCDs in turn)
 };
 constexpr int kBarWords = 4;
-// points one pass of a plane task handles: 16 wavefronts of 7 groups of 9 lanes
-constexpr int kForcePlanePoints = 112;
+// points one pass of a plane task handles: 16 wavefronts of 21 groups of 3 lanes
+constexpr int kForcePlanePoints = 336;
 constexpr int kForcePlaneBlock = 1024;  // threads of a plane task's workgroup, at most
 // Workgroup of a batch's forcing planes for a mean of `cells` static stencil
 // cells per task: a lane per ~4 cells, a power of two from 128 to

@@ -1224,27 +1224,30 @@ __global__ __launch_bounds__(kBlock) void k_ib_moments_points(PointsDev p, int64
     cache[t] = m;
 }
 
-// Same arithmetic as k_force_level, one point per 9 lanes of a wavefront (lane
-// s of the group holds stencil cell s = (k1+1)*3 + (i1+1)), so the gathers of a
-// point run in parallel.  Stencil cell s reads cache[sidx[9q+s]] (INDEXED) or
-// cache[9(q-cache_base)+s].  Every lane of the group adds the 9 terms in the
-// reference's order (s = 0..8, each product rounded before the add), so the
-// sums are those of the sequential loop bit for bit.  The 9 cells of a point
-// are distinct (the cached path needs 3 interior cells in x and z) and the
-// points of a level have disjoint stencils, so each F entry has one writer.
-constexpr int kGroup = 9;                  // lanes per point
-constexpr int kGroupsPerWave = 64 / kGroup;  // 7 points per wavefront, lane 63 idle
+// Same arithmetic as k_force_level, one point per 3 lanes of a wavefront: lane
+// c of the group holds the stencil row k1 = c - 1, its cells s = 3c + m for
+// i1 = m - 1 (s = (k1+1)*3 + (i1+1)), so the 9 gathers of a point run in
+// parallel, three per lane.  Stencil cell s reads cache[sidx[9q+s]] (INDEXED)
+// or cache[9(q-cache_base)+s].  Every lane of the group adds the 9 terms in
+// the reference's order (s = 0..8, each product rounded before the add), so
+// the sums are those of the sequential loop bit for bit.  The 9 cells of a
+// point are distinct (the cached path needs 3 interior cells in x and z) and
+// the points of a level have disjoint stencils, so each F entry has one writer.
+// Three lanes per point put 21 points in a wavefront (7 with a lane per cell):
+// a level's passes, each a chain of dependent loads, are a third as many.
+constexpr int kGroup = 3;                    // lanes per point
+constexpr int kGroupsPerWave = 64 / kGroup;  // 21 points per wavefront, lane 63 idle
 
-// The inputs of stencil cell s of point q, as loaded (force_point9 in two
+// The inputs of stencil row c of point q, as loaded (force_point9 in two
 // parts, so that a pass can load the next pass's points while it computes).
 template <typename T>
 struct PointRaw {
     int pi, pj, pk;
     T ifr, kfr, u, v, w;
-    int64_t e;  // cache entry
+    int64_t e[3];  // cache entries of the row's cells
 };
 template <typename T, bool INDEXED>
-__device__ __forceinline__ PointRaw<T> point_raw(const PointsDev& p, int64_t q, bool valid, int s,
+__device__ __forceinline__ PointRaw<T> point_raw(const PointsDev& p, int64_t q, bool valid, int c,
                                                  const int32_t* __restrict__ sidx, int64_t cache_base) {
     PointRaw<T> r{};
     if (valid) {
@@ -1256,65 +1259,86 @@ __device__ __forceinline__ PointRaw<T> point_raw(const PointsDev& p, int64_t q, 
         r.u = static_cast<const T*>(p.u)[q];
         r.v = static_cast<const T*>(p.v)[q];
         r.w = static_cast<const T*>(p.w)[q];
-        r.e = INDEXED ? (int64_t)sidx[9 * q + s] : 9 * (q - cache_base) + s;
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+            r.e[m] = INDEXED ? (int64_t)sidx[9 * q + 3 * c + m] : 9 * (q - cache_base) + 3 * c + m;
     }
     return r;
 }
 
 template <typename T, typename D>
-__device__ __forceinline__ void force_point9_raw(const PointRaw<T>& r, bool valid, int s, int base,
+__device__ __forceinline__ void force_point9_raw(const PointRaw<T>& r, bool valid, int c, int base,
                                                  const Mom4<T>* __restrict__ cache, T* __restrict__ F, const Grid& g,
                                                  T alfa, T beta) {
-    T w = T(0), tr = T(0), tx = T(0), ty = T(0), tz = T(0), fo0 = T(0), fo1 = T(0), fo2 = T(0);
-    uint64_t cell = 0;
-    if (valid) {
-        const int a = s % 3, b = s / 3;
-        w = ib_weight<T>(-r.ifr + T(a - 1)) * ib_weight<T>(-r.kfr + T(b - 1));
-        const int i2 = wrap_ik(r.pi + a - 1, g.xg1), k2 = wrap_ik(r.pk + b - 1, g.zg1);
-        cell = cell_index(g, i2, r.pj, k2);
-        const Mom4<T> m = cache[r.e];
-        fo0 = F[3 * cell + 0];
-        fo1 = F[3 * cell + 1];
-        fo2 = F[3 * cell + 2];
-        const T xx = (T)((D)m.x + D(0.5) * (D)fo0);
-        const T yy = (T)((D)m.y + D(0.5) * (D)fo1);
-        const T zz = (T)((D)m.z + D(0.5) * (D)fo2);
-        tr = w * m.r;
-        tx = w * xx;
-        ty = w * yy;
-        tz = w * zz;
+    T w[3], tr[3], tx[3], ty[3], tz[3], fo[3][3];
+    uint64_t cell[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        w[m] = tr[m] = tx[m] = ty[m] = tz[m] = fo[m][0] = fo[m][1] = fo[m][2] = T(0);
+        cell[m] = 0;
     }
+    if (valid) {
+        const T wk = ib_weight<T>(-r.kfr + T(c - 1));
+        const int k2 = wrap_ik(r.pk + c - 1, g.zg1);
+        Mom4<T> mo[3];
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            const int i2 = wrap_ik(r.pi + m - 1, g.xg1);
+            cell[m] = cell_index(g, i2, r.pj, k2);
+            mo[m] = cache[r.e[m]];
+            fo[m][0] = F[3 * cell[m] + 0];
+            fo[m][1] = F[3 * cell[m] + 1];
+            fo[m][2] = F[3 * cell[m] + 2];
+        }
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            w[m] = ib_weight<T>(-r.ifr + T(m - 1)) * wk;
+            const T xx = (T)((D)mo[m].x + D(0.5) * (D)fo[m][0]);
+            const T yy = (T)((D)mo[m].y + D(0.5) * (D)fo[m][1]);
+            const T zz = (T)((D)mo[m].z + D(0.5) * (D)fo[m][2]);
+            tr[m] = w[m] * mo[m].r;
+            tx[m] = w[m] * xx;
+            ty[m] = w[m] * yy;
+            tz[m] = w[m] * zz;
+        }
+    }
+    // terms s = 3 c2 + m in the order s = 0..8: row c2 from lane base + c2
     T rhoSum = T(0.0), xSum = T(0.0), ySum = T(0.0), zSum = T(0.0);
 #pragma unroll
-    for (int t = 0; t < kGroup; t++) {
-        rhoSum += __shfl(tr, base + t, 64);
-        xSum += __shfl(tx, base + t, 64);
-        ySum += __shfl(ty, base + t, 64);
-        zSum += __shfl(tz, base + t, 64);
-    }
+    for (int c2 = 0; c2 < kGroup; c2++)
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            rhoSum += __shfl(tr[m], base + c2, 64);
+            xSum += __shfl(tx[m], base + c2, 64);
+            ySum += __shfl(ty[m], base + c2, 64);
+            zSum += __shfl(tz[m], base + c2, 64);
+        }
     if (!valid) return;
     xSum -= rhoSum * r.u;
     ySum -= rhoSum * r.v;
     zSum -= rhoSum * r.w;
-    const T bp = beta * w;
-    F[3 * cell + 0] = alfa * fo0 - bp * xSum;
-    F[3 * cell + 1] = alfa * fo1 - bp * ySum;
-    F[3 * cell + 2] = alfa * fo2 - bp * zSum;
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        const T bp = beta * w[m];
+        F[3 * cell[m] + 0] = alfa * fo[m][0] - bp * xSum;
+        F[3 * cell[m] + 1] = alfa * fo[m][1] - bp * ySum;
+        F[3 * cell[m] + 2] = alfa * fo[m][2] - bp * zSum;
+    }
 }
 
 template <typename T, typename D, bool INDEXED>
-__device__ __forceinline__ void force_point9(const PointsDev& p, int64_t q, bool valid, int s, int base,
+__device__ __forceinline__ void force_point9(const PointsDev& p, int64_t q, bool valid, int c, int base,
                                              const int32_t* __restrict__ sidx, int64_t cache_base,
                                              const Mom4<T>* __restrict__ cache, T* __restrict__ F, const Grid& g,
                                              T alfa, T beta) {
-    const PointRaw<T> r = point_raw<T, INDEXED>(p, q, valid, s, sidx, cache_base);
-    force_point9_raw<T, D>(r, valid, s, base, cache, F, g, alfa, beta);
+    const PointRaw<T> r = point_raw<T, INDEXED>(p, q, valid, c, sidx, cache_base);
+    force_point9_raw<T, D>(r, valid, c, base, cache, F, g, alfa, beta);
 }
 
-// lane -> (point index within the launch, stencil cell, group base lane)
-__device__ __forceinline__ int64_t group_point(int& s, int& base) {
+// lane -> (point index within the launch, stencil row, group base lane)
+__device__ __forceinline__ int64_t group_point(int& c, int& base) {
     const int lane = (int)(threadIdx.x & 63), grp = lane / kGroup;
-    s = lane - grp * kGroup;
+    c = lane - grp * kGroup;
     base = grp * kGroup;
     const int64_t wave = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
     return grp < kGroupsPerWave ? wave * kGroupsPerWave + grp : -1;
@@ -1344,7 +1368,7 @@ __global__ __launch_bounds__(kBlock) void k_force_level_cached(PointsDev p, int6
 // stencil cells missing from the previous mask (prepare_point) and computes the
 // density and momentum of its static and blade stencil cells; then it runs its
 // static levels and its blade colour classes in order, passes of up to
-// kPlanePoints points per workgroup, 9 lanes per point (force_point9), with a
+// kPlanePoints points per workgroup, 3 lanes per point (force_point9), with a
 // barrier after every level: the sequential list order of Forcing.hpp:73-135
 // within the plane, without a kernel launch per level.  A plane with many
 // points runs on a team of workgroups that meet at a barrier in global memory
@@ -1362,15 +1386,21 @@ constexpr uint32_t kBarSpinLimit = 1u << 20;
 // bar[1] is the generation, bar[2] the time-out flag.  The last arrival resets
 // the count (waiting for that reset to complete) and then advances the
 // generation, so the barrier is ready for the next level and the next launch.
-// Local teams (every member on one XCD, ForceGeom::local_teams) exchange F and
-// the caches through that XCD's L2: each wave's stores complete before the
-// arrival, and each CU drops its L1 after the wait; the last arrival checks
-// that every member reported the same XCC id (bar[2] bit 2 otherwise).  Other teams fence at agent
-// scope (L2 write-back before the arrival, invalidation after the wait).  The
-// wait polls with relaxed loads (an acquire load would invalidate the L2 on
-// every poll).
+// Every wave waits for its own stores before the workgroup barrier that
+// precedes the arrival: the barrier itself does not wait for them, and the
+// arriving thread's fence covers only its own wave's stores, so a store of
+// another wave could otherwise still be on its way to another L2 channel when
+// the arrival is seen.  Local teams (every member on one XCD,
+// ForceGeom::local_teams) exchange F and the caches through that XCD's L2, and
+// each CU drops its L1 after the wait, with an invalidate at device scope (at
+// work-group scope it leaves the L1 alone when work-groups are not split across
+// CUs, and a member would read back F lines it cached before another member
+// wrote them); the last arrival checks that every member reported the same XCC
+// id (bar[2] bit 2 otherwise).  Other teams fence at agent scope (L2
+// write-back before the arrival, invalidation after the wait).  The wait polls
+// with relaxed loads (an acquire load would invalidate the L2 on every poll).
 __device__ __forceinline__ void team_barrier(uint32_t* bar, int team, bool local) {
-    if (local) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (threadIdx.x == 0) {
         if (!local) __threadfence();
@@ -1403,13 +1433,13 @@ __device__ __forceinline__ void team_barrier(uint32_t* bar, int team, bool local
         if (!local) __threadfence();
     }
     __syncthreads();
-    if (local) asm volatile("buffer_inv sc0" ::: "memory");
+    if (local) asm volatile("buffer_inv sc1" ::: "memory");
 }
 
-// lane -> point slot of a pass (wave-major, 7 groups of 9 lanes per wave), -1 for lane 63
-__device__ __forceinline__ int plane_slot(int& s, int& base) {
+// lane -> point slot of a pass (wave-major, 21 groups of 3 lanes per wave), -1 for lane 63
+__device__ __forceinline__ int plane_slot(int& c, int& base) {
     const int lane = (int)(threadIdx.x & 63), grp = lane / kGroup;
-    s = lane - grp * kGroup;
+    c = lane - grp * kGroup;
     base = grp * kGroup;
     return grp < kGroupsPerWave ? (int)(threadIdx.x >> 6) * kGroupsPerWave + grp : -1;
 }
@@ -1509,8 +1539,8 @@ __device__ void plane_levels(const ForceGeom& fg, const PlaneDesc& pd, int rank,
                              const Grid& g, T alfa, T beta, Sync&& sync) {
     const auto* scache = static_cast<const Mom4<T>*>(fg.scache);
     const auto* rcache = static_cast<const Mom4<T>*>(fg.rcache);
-    int s, gbase;
-    const int slot = plane_slot(s, gbase);
+    int row, gbase;
+    const int slot = plane_slot(row, gbase);
     const int64_t per = (int64_t)(blockDim.x >> 6) * kGroupsPerWave;
     const int64_t p0 = (int64_t)rank * per, pstride = (int64_t)team * per;
     // Each pass loads the next pass's points (positions, fractions, velocities,
@@ -1522,7 +1552,7 @@ __device__ void plane_levels(const ForceGeom& fg, const PlaneDesc& pd, int rank,
     auto first = [&](int lv) { return fg.lev_off[pd.lev + lv] + p0; };
     auto raw_at = [&](int lv, int64_t qq) {
         const bool ok = lv < nlev && slot >= 0 && qq + slot < lev_end(lv);
-        return point_raw<T, true>(fg.pts, ok ? qq + slot : 0, ok, s, fg.sidx, 0);
+        return point_raw<T, true>(fg.pts, ok ? qq + slot : 0, ok, row, fg.sidx, 0);
     };
     int l = 0;
     while (l < nlev && first(l) >= lev_end(l)) {
@@ -1541,7 +1571,7 @@ __device__ void plane_levels(const ForceGeom& fg, const PlaneDesc& pd, int rank,
             nq = nl < nlev ? first(nl) : 0;
         }
         const PointRaw<T> nxt = raw_at(nl, nq);
-        force_point9_raw<T, D>(cur, slot >= 0 && q + slot < lev_end(l), s, gbase, scache, F, g, alfa, beta);
+        force_point9_raw<T, D>(cur, slot >= 0 && q + slot < lev_end(l), row, gbase, scache, F, g, alfa, beta);
         for (int k = l; k < nl; k++) sync();  // levels l .. nl - 1 are done
         l = nl;
         q = nq;
@@ -1562,7 +1592,7 @@ __device__ void plane_levels(const ForceGeom& fg, const PlaneDesc& pd, int rank,
                 const int bi = (int)(t / bc.nr), ri = (int)(t - (int64_t)bi * bc.nr);
                 qb += bc.base + ((int64_t)bi * fg.blade.n_planes + bp) * bc.nr + ri;
             }
-            force_point9<T, D, false>(fg.pts, qb, valid, s, gbase, nullptr, fg.n_static, rcache, F, g, alfa, beta);
+            force_point9<T, D, false>(fg.pts, qb, valid, row, gbase, nullptr, fg.n_static, rcache, F, g, alfa, beta);
         }
         sync();
         base += blade_class_size(be, c);

@@ -266,6 +266,29 @@ def test_gpu_one_launch_step_equals_separate_launches(prec, monkeypatch):
         np.testing.assert_array_equal(a, b)
 
 
+def test_gpu_disk_plane_team_at_295_matches_oracle():
+    """A 295^3 tank: the impeller disk's plane runs on a team whose levels
+    overflow one pass of a workgroup, so team members write F lines that
+    another member has cached; the forcing at bind and after one step is
+    bit-identical to the oracle (the members' L1 is dropped at the barrier)."""
+    T = 295
+    case(f"disk plane team T={T} bind + 1 step Esotwist/LIJK fp32")
+    oflow, gflow = flow_for_tank(T)
+    o = O.OracleCU(T, dtype=O.F32, layout=O.LIJK, streaming=O.ESOTWIST, use_les=True, **oflow)
+    g = td.ComputeUnit(T, layout=td.LAYOUT_LIJK, streaming=td.STREAM_ESOTWIST, precision=td.PREC_F32, **gflow)
+    rp = td.rushton_defaults(T)
+    o.initialise(1.0)
+    g.initialise(1.0)
+    ref = ReferenceLoop(o, rp)
+    g.bind_rushton(rp, step=1, angle=0.0)
+    np.testing.assert_array_equal(g.get(td.FIELD_F), o.get(O.FIELD_F))
+    ref.run(1)
+    g.run(1)
+    g.synchronize()
+    np.testing.assert_array_equal(g.get(td.FIELD_F), o.get(O.FIELD_F))
+    g.close()
+
+
 @pytest.mark.parametrize("stream", STREAMS, ids=["Simple", "Esotwist"])
 def test_gpu_forcing_plane_teams_match_oracle(stream, monkeypatch):
     """Planes forced by teams of workgroups meeting at a barrier in global memory
