@@ -160,11 +160,11 @@ constexpr int kForcePlanePoints = 336;
 constexpr int kForcePlaneBlock = 1024;  // threads of a plane task's workgroup, at most
 // Workgroup of a batch's forcing planes for a mean of `cells` static stencil
 // cells per task: a lane per ~4 cells, a power of two from 128 to
-// kForcePlaneBlock lanes (batches of 48^3 tanks: 128 lanes 1.6x faster than
-// 1024; the mixed 48..160 batch: 256 lanes 4 % faster; 160^3: 1024 lanes 1 %
-// faster).  A batch has thousands of tasks; a single tank's few hundred
-// planes leave CUs idle anyway and run faster with 1024 lanes (96^3: 0.091
-// against 0.123 ms per pass).
+// kForcePlaneBlock lanes (batches of 48^3 tanks: 128 lanes 1.7x faster than
+// 1024; the mixed 48..160 batch: 256 lanes 9 % faster; 160^3 tanks: 256
+// lanes 2 % faster, 512 lanes 4 %).  A batch has thousands of tasks; a single
+// tank's few hundred planes leave CUs idle anyway and run faster with 1024
+// lanes (512^3: 0.653 against 0.734 ms per pass with 512 lanes).
 inline int forcing_block_for(double cells) {
     int t = 128;
     while (t < kForcePlaneBlock && 2.0 * t <= cells / 4.0) t *= 2;
