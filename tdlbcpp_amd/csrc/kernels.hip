@@ -1236,7 +1236,11 @@ __global__ __launch_bounds__(kBlock) void k_ib_moments_points(PointsDev p, int64
 // Three lanes per point put 21 points in a wavefront (7 with a lane per cell):
 // a level's passes, each a chain of dependent loads, are a third as many.
 constexpr int kGroup = 3;                    // lanes per point
-constexpr int kGroupsPerWave = 64 / kGroup;  // 21 points per wavefront, lane 63 idle
+constexpr int kGroupsPerWave = 64 / kGroup;
+// LDS of a single tank's plane-task workgroup for the force values of a solo
+// plane's static cells: 8192 cells in FP32 (6570 per plane at 512^3).  The
+// workgroup's registers already hold a CU to one workgroup.
+constexpr size_t kPlaneLdsBytes = 96 * 1024;  // 21 points per wavefront, lane 63 idle
 
 // The inputs of stencil row c of point q, as loaded (force_point9 in two
 // parts, so that a pass can load the next pass's points while it computes).
@@ -1323,6 +1327,65 @@ __device__ __forceinline__ void force_point9_raw(const PointRaw<T>& r, bool vali
         F[3 * cell[m] + 0] = alfa * fo[m][0] - bp * xSum;
         F[3 * cell[m] + 1] = alfa * fo[m][1] - bp * ySum;
         F[3 * cell[m] + 2] = alfa * fo[m][2] - bp * zSum;
+    }
+}
+
+// force_point9_raw with the row's force values in LDS: entry e of the
+// plane's static cells at lds[3 (e - cb)], the same arithmetic in the same order.
+template <typename T, typename D>
+__device__ __forceinline__ void force_point9_lds(const PointRaw<T>& r, bool valid, int c, int base,
+                                                 const Mom4<T>* __restrict__ cache, T* lds, int64_t cb, T alfa,
+                                                 T beta) {
+    T w[3], tr[3], tx[3], ty[3], tz[3], fo[3][3];
+    int li[3];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        w[m] = tr[m] = tx[m] = ty[m] = tz[m] = fo[m][0] = fo[m][1] = fo[m][2] = T(0);
+        li[m] = 0;
+    }
+    if (valid) {
+        const T wk = ib_weight<T>(-r.kfr + T(c - 1));
+        Mom4<T> mo[3];
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            li[m] = 3 * (int)(r.e[m] - cb);
+            mo[m] = cache[r.e[m]];
+            fo[m][0] = lds[li[m] + 0];
+            fo[m][1] = lds[li[m] + 1];
+            fo[m][2] = lds[li[m] + 2];
+        }
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            w[m] = ib_weight<T>(-r.ifr + T(m - 1)) * wk;
+            const T xx = (T)((D)mo[m].x + D(0.5) * (D)fo[m][0]);
+            const T yy = (T)((D)mo[m].y + D(0.5) * (D)fo[m][1]);
+            const T zz = (T)((D)mo[m].z + D(0.5) * (D)fo[m][2]);
+            tr[m] = w[m] * mo[m].r;
+            tx[m] = w[m] * xx;
+            ty[m] = w[m] * yy;
+            tz[m] = w[m] * zz;
+        }
+    }
+    T rhoSum = T(0.0), xSum = T(0.0), ySum = T(0.0), zSum = T(0.0);
+#pragma unroll
+    for (int c2 = 0; c2 < kGroup; c2++)
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            rhoSum += __shfl(tr[m], base + c2, 64);
+            xSum += __shfl(tx[m], base + c2, 64);
+            ySum += __shfl(ty[m], base + c2, 64);
+            zSum += __shfl(tz[m], base + c2, 64);
+        }
+    if (!valid) return;
+    xSum -= rhoSum * r.u;
+    ySum -= rhoSum * r.v;
+    zSum -= rhoSum * r.w;
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        const T bp = beta * w[m];
+        lds[li[m] + 0] = alfa * fo[m][0] - bp * xSum;
+        lds[li[m] + 1] = alfa * fo[m][1] - bp * ySum;
+        lds[li[m] + 2] = alfa * fo[m][2] - bp * zSum;
     }
 }
 
@@ -1536,7 +1599,7 @@ __device__ void gather_blades(const ForceGeom& fg, int j, int64_t t0, int64_t th
 // sync() after every level.
 template <typename T, typename D, typename Sync>
 __device__ void plane_levels(const ForceGeom& fg, const PlaneDesc& pd, int rank, int team, T* __restrict__ F,
-                             const Grid& g, T alfa, T beta, Sync&& sync) {
+                             const Grid& g, T alfa, T beta, Sync&& sync, T* lds = nullptr) {
     const auto* scache = static_cast<const Mom4<T>*>(fg.scache);
     const auto* rcache = static_cast<const Mom4<T>*>(fg.rcache);
     int row, gbase;
@@ -1571,11 +1634,27 @@ __device__ void plane_levels(const ForceGeom& fg, const PlaneDesc& pd, int rank,
             nq = nl < nlev ? first(nl) : 0;
         }
         const PointRaw<T> nxt = raw_at(nl, nq);
-        force_point9_raw<T, D>(cur, slot >= 0 && q + slot < lev_end(l), row, gbase, scache, F, g, alfa, beta);
+        if (lds)
+            force_point9_lds<T, D>(cur, slot >= 0 && q + slot < lev_end(l), row, gbase, scache, lds, pd.cell_begin,
+                                   alfa, beta);
+        else
+            force_point9_raw<T, D>(cur, slot >= 0 && q + slot < lev_end(l), row, gbase, scache, F, g, alfa, beta);
         for (int k = l; k < nl; k++) sync();  // levels l .. nl - 1 are done
         l = nl;
         q = nq;
         cur = nxt;
+    }
+    if (lds) {
+        // the static levels are done (each ended with a barrier): their force
+        // values back to F before the blade classes read it
+        for (int64_t e = threadIdx.x; e < pd.cell_end - pd.cell_begin; e += blockDim.x) {
+            const uint64_t cc = fg.cells[pd.cell_begin + e];  // cell_index of the entry
+            F[3 * cc + 0] = lds[3 * e + 0];
+            F[3 * cc + 1] = lds[3 * e + 1];
+            F[3 * cc + 2] = lds[3 * e + 2];
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
     }
     int ncls;
     const int bp = blade_plane(fg, pd.j, ncls);
@@ -1604,7 +1683,8 @@ template <typename PR, int L, int STREAM, int MOMSRC, int EVEN>
 __device__ void force_plane(const ForceGeom& fg, const PlaneDesc& pd, int rank,
                             const typename PR::S* __restrict__ Q, typename PR::T* __restrict__ F,
                             const uint64_t* __restrict__ prev, uint64_t* __restrict__ next, const Grid& g,
-                            typename PR::T alfa, typename PR::T beta, float rho0) {
+                            typename PR::T alfa, typename PR::T beta, float rho0,
+                            typename PR::T* lds_buf = nullptr, int64_t lds_cells = 0) {
     const int team = pd.team;
     uint32_t* bar = team > 1 ? fg.bar + kBarWords * pd.bar : nullptr;
     const int64_t t0 = (int64_t)rank * blockDim.x + threadIdx.x, threads = (int64_t)team * blockDim.x;
@@ -1616,7 +1696,23 @@ __device__ void force_plane(const ForceGeom& fg, const PlaneDesc& pd, int rank,
     gather_cells<PR, L, STREAM, MOMSRC, EVEN>(fg, pd.cell_begin, pd.cell_end, t0, threads, Q, g, rho0);
     gather_blades<PR, L, STREAM, MOMSRC, EVEN>(fg, pd.j, t0, threads, Q, F, prev, next, g, rho0);
     sync();
-    plane_levels<typename PR::T, typename PR::D>(fg, pd, rank, team, F, g, alfa, beta, sync);
+    // A solo plane whose static cells fit keeps their force values in LDS
+    // through the static levels (every stencil cell of a plane task lies in
+    // its own plane, which no other task touches): each level then waits on
+    // LDS instead of on F in HBM.  Loaded after the blade cells' zeroing.
+    using T = typename PR::T;
+    const int64_t nc = pd.cell_end - pd.cell_begin;
+    T* lds = (lds_buf && team == 1 && nc <= lds_cells) ? lds_buf : nullptr;
+    if (lds) {
+        for (int64_t e = threadIdx.x; e < nc; e += blockDim.x) {
+            const uint64_t cc = fg.cells[pd.cell_begin + e];  // cell_index of the entry
+            lds[3 * e + 0] = F[3 * cc + 0];
+            lds[3 * e + 1] = F[3 * cc + 1];
+            lds[3 * e + 2] = F[3 * cc + 2];
+        }
+        __syncthreads();
+    }
+    plane_levels<T, typename PR::D>(fg, pd, rank, team, F, g, alfa, beta, sync, lds);
 }
 
 template <typename PR, int L, int STREAM, int MOMSRC, int EVEN>
@@ -1625,10 +1721,13 @@ __global__ __launch_bounds__(kPlaneBlock) void k_force_planes(ForceGeom fg, cons
                                                               const uint64_t* __restrict__ prev,
                                                               uint64_t* __restrict__ next, Grid g,
                                                               typename PR::T alfa, typename PR::T beta, float rho0) {
+    __shared__ float4 lds_f[kPlaneLdsBytes / sizeof(float4)];
     const PlaneBlock pb = fg.blocks[blockIdx.x];
     if (pb.plane < 0) return;
+    using T = typename PR::T;
     force_plane<PR, L, STREAM, MOMSRC, EVEN>(fg, fg.planes[pb.plane], pb.rank, Q, F, prev, next, g, alfa, beta,
-                                             rho0);
+                                             rho0, reinterpret_cast<T*>(lds_f),
+                                             (int64_t)(kPlaneLdsBytes / (3 * sizeof(T))));
 }
 
 // The levels of every plane after the gathers of the one-launch step: a
