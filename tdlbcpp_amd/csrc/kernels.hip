@@ -1600,6 +1600,7 @@ __device__ void gather_blades(const ForceGeom& fg, int j, int64_t t0, int64_t th
 template <typename T, typename D, typename Sync>
 __device__ void plane_levels(const ForceGeom& fg, const PlaneDesc& pd, int rank, int team, T* __restrict__ F,
                              const Grid& g, T alfa, T beta, Sync&& sync, T* lds = nullptr) {
+    if constexpr (sizeof(T) > 4) lds = nullptr;  // FP64: the registers of the LDS form spill
     const auto* scache = static_cast<const Mom4<T>*>(fg.scache);
     const auto* rcache = static_cast<const Mom4<T>*>(fg.rcache);
     int row, gbase;
@@ -1702,7 +1703,7 @@ __device__ void force_plane(const ForceGeom& fg, const PlaneDesc& pd, int rank,
     // LDS instead of on F in HBM.  Loaded after the blade cells' zeroing.
     using T = typename PR::T;
     const int64_t nc = pd.cell_end - pd.cell_begin;
-    T* lds = (lds_buf && team == 1 && nc <= lds_cells) ? lds_buf : nullptr;
+    T* lds = (sizeof(T) <= 4 && lds_buf && team == 1 && nc <= lds_cells) ? lds_buf : nullptr;
     if (lds) {
         for (int64_t e = threadIdx.x; e < nc; e += blockDim.x) {
             const uint64_t cc = fg.cells[pd.cell_begin + e];  // cell_index of the entry
