@@ -300,6 +300,7 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     // must be resident together), then every solo plane, longest first
     std::vector<PlaneTask> team_tasks;
     std::vector<std::pair<float, PlaneTask>> tk;
+    int64_t solo_cells = 0;  // static cells of the largest solo plane
     for (int r = 0; r < M; r++) {
         Member& m = b->members[idx[r]];
         tdlb_cu* cu = m.cu;
@@ -311,8 +312,13 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
         const auto& pl = cu->static_set.plane_list;
         for (int32_t b = 0; b < (int32_t)bl.size(); b++) {
             if (bl[b].plane < 0) continue;  // an empty slot of the single-tank launch
-            if (pl[bl[b].plane].team > 1) team_tasks.push_back(PlaneTask{r, b});
-            else tk.push_back({cu->static_set.plane_cost[bl[b].plane], PlaneTask{r, b}});
+            const PlaneDesc& pd = pl[bl[b].plane];
+            if (pd.team > 1) {
+                team_tasks.push_back(PlaneTask{r, b});
+            } else {
+                tk.push_back({cu->static_set.plane_cost[bl[b].plane], PlaneTask{r, b}});
+                solo_cells = std::max<int64_t>(solo_cells, pd.cell_end - pd.cell_begin);
+            }
         }
         // blade orientation of each step (the single-handle loop computes the same)
         Rushton R(cu->rp);
@@ -389,6 +395,16 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
         const int t = std::atoi(v);
         if (t >= 64 && t <= kForcePlaneBlock && t % 64 == 0) p.force_bs = t;
     }
+    // The force values of a solo plane stay in LDS through its levels when they
+    // fit a share of the CU's LDS that keeps the workgroups resident together
+    // that the registers allow (at most 128 VGPRs: 1024 lanes per CU); larger
+    // planes keep them in HBM.  All batch formats compute in 4-byte floats.
+    // The launch-sized LDS stays within 64 KB, which needs no kernel attribute.
+    {
+        const int64_t per_cu = std::max<int64_t>(1, kForcePlaneBlock / p.force_bs);
+        const int64_t bytes = (int64_t)std::min<size_t>({kPlaneLdsBytes, kCuLdsBytes / (size_t)per_cu, 64 * 1024});
+        p.force_lds_cells = std::min<int64_t>(solo_cells, bytes / (3 * (int64_t)sizeof(float)));
+    }
     p.tot_coll = tot_coll;
     p.tot_rot = tot_rot;
     p.tot_words = tot_words;
@@ -399,7 +415,7 @@ int build_plan(tdlb_batch* b, int64_t K, bool* structure_changed) {
     // A captured step bakes in every launch size and kernel argument: recapture
     // whenever any of them differs (member identity alone is not enough, a freed
     // handle's address can come back for a tank of another size).
-    std::vector<int64_t> sig = {M, tot_coll, tot_rot, tot_words, p.n_tasks, p.bs, p.force_bs,
+    std::vector<int64_t> sig = {M, tot_coll, tot_rot, tot_words, p.n_tasks, p.bs, p.force_bs, p.force_lds_cells,
                                 (int64_t)(uintptr_t)b->d_mem.p, (int64_t)(uintptr_t)b->d_tab.p,
                                 (int64_t)(uintptr_t)b->d_tasks.p, (int64_t)(uintptr_t)b->d_bm.p};
     *structure_changed = moved || sig != b->launch_sig || key != b->key || b->buffers_moved;

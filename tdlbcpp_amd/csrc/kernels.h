@@ -158,6 +158,18 @@ constexpr int kBarWords = 4;
 // points one pass of a plane task handles: 16 wavefronts of 21 groups of 3 lanes
 constexpr int kForcePlanePoints = 336;
 constexpr int kForcePlaneBlock = 1024;  // threads of a plane task's workgroup, at most
+// LDS of a single tank's plane-task workgroup for the force values of a solo
+// plane's static cells: 8192 cells in FP32 (6570 per plane at 512^3).  The
+// workgroup's registers already hold a CU to one workgroup.
+constexpr size_t kPlaneLdsBytes = 96 * 1024;
+constexpr size_t kCuLdsBytes = 160 * 1024;  // LDS of a gfx950 CU
+// A single tank's plane keeps its force values in LDS only from this many
+// static cells on: with few planes per CU (tanks to about 256^3) the pass is
+// not bandwidth-bound and the copy in and out costs more than the levels save
+// (64 tanks of 48..160^3 one at a time: 3.5 % slower with every plane in LDS;
+// 512^3, 6570 cells per plane: 4.5 % faster).  Batches, whose thousands of
+// tasks keep every CU gathering, use LDS for every plane that fits.
+constexpr int64_t kPlaneLdsMinCells = 4096;
 // Workgroup of a batch's forcing planes for a mean of `cells` static stencil
 // cells per task: a lane per ~4 cells, a power of two from 128 to
 // kForcePlaneBlock lanes (batches of 48^3 tanks: 128 lanes 1.7x faster than
@@ -318,6 +330,7 @@ struct BatchPlan {
     int64_t tot_coll, tot_rot, tot_words, n_tasks;
     int32_t bs;  // threads per block of the fused kernel (256 or 512)
     int32_t force_bs;  // threads per workgroup of the forcing pass (64 .. kForcePlaneBlock, a multiple of 64)
+    int64_t force_lds_cells;  // force values of a solo plane task kept in LDS, at most (0: none)
 };
 
 // Enqueue `steps` main-loop steps of every active member: per step blade

@@ -1236,11 +1236,7 @@ __global__ __launch_bounds__(kBlock) void k_ib_moments_points(PointsDev p, int64
 // Three lanes per point put 21 points in a wavefront (7 with a lane per cell):
 // a level's passes, each a chain of dependent loads, are a third as many.
 constexpr int kGroup = 3;                    // lanes per point
-constexpr int kGroupsPerWave = 64 / kGroup;
-// LDS of a single tank's plane-task workgroup for the force values of a solo
-// plane's static cells: 8192 cells in FP32 (6570 per plane at 512^3).  The
-// workgroup's registers already hold a CU to one workgroup.
-constexpr size_t kPlaneLdsBytes = 96 * 1024;  // 21 points per wavefront, lane 63 idle
+constexpr int kGroupsPerWave = 64 / kGroup;  // 21 points per wavefront, lane 63 idle
 
 // The inputs of stencil row c of point q, as loaded (force_point9 in two
 // parts, so that a pass can load the next pass's points while it computes).
@@ -1685,7 +1681,7 @@ __device__ void force_plane(const ForceGeom& fg, const PlaneDesc& pd, int rank,
                             const typename PR::S* __restrict__ Q, typename PR::T* __restrict__ F,
                             const uint64_t* __restrict__ prev, uint64_t* __restrict__ next, const Grid& g,
                             typename PR::T alfa, typename PR::T beta, float rho0,
-                            typename PR::T* lds_buf = nullptr, int64_t lds_cells = 0) {
+                            typename PR::T* lds_buf = nullptr, int64_t lds_cells = 0, int64_t lds_min = 0) {
     const int team = pd.team;
     uint32_t* bar = team > 1 ? fg.bar + kBarWords * pd.bar : nullptr;
     const int64_t t0 = (int64_t)rank * blockDim.x + threadIdx.x, threads = (int64_t)team * blockDim.x;
@@ -1703,7 +1699,7 @@ __device__ void force_plane(const ForceGeom& fg, const PlaneDesc& pd, int rank,
     // LDS instead of on F in HBM.  Loaded after the blade cells' zeroing.
     using T = typename PR::T;
     const int64_t nc = pd.cell_end - pd.cell_begin;
-    T* lds = (sizeof(T) <= 4 && lds_buf && team == 1 && nc <= lds_cells) ? lds_buf : nullptr;
+    T* lds = (sizeof(T) <= 4 && lds_buf && team == 1 && nc <= lds_cells && nc >= lds_min) ? lds_buf : nullptr;
     if (lds) {
         for (int64_t e = threadIdx.x; e < nc; e += blockDim.x) {
             const uint64_t cc = fg.cells[pd.cell_begin + e];  // cell_index of the entry
@@ -1728,7 +1724,7 @@ __global__ __launch_bounds__(kPlaneBlock) void k_force_planes(ForceGeom fg, cons
     using T = typename PR::T;
     force_plane<PR, L, STREAM, MOMSRC, EVEN>(fg, fg.planes[pb.plane], pb.rank, Q, F, prev, next, g, alfa, beta,
                                              rho0, reinterpret_cast<T*>(lds_f),
-                                             (int64_t)(kPlaneLdsBytes / (3 * sizeof(T))));
+                                             (int64_t)(kPlaneLdsBytes / (3 * sizeof(T))), kPlaneLdsMinCells);
 }
 
 // The levels of every plane after the gathers of the one-launch step: a
@@ -3081,11 +3077,15 @@ __global__ void k_batch_mask_init(const MemberDev* __restrict__ mem, const int64
 // density and momentum from the populations the fused kernel just wrote
 template <typename PR, int L, int STREAM>
 __global__ __launch_bounds__(kPlaneBlock) void k_batch_force_planes(const MemberDev* __restrict__ mem,
-                                                                    const PlaneTask* __restrict__ tasks) {
+                                                                    const PlaneTask* __restrict__ tasks,
+                                                                    int64_t lds_cells) {
+    extern __shared__ float4 batch_lds[];  // lds_cells force values (3 T each), launch-sized
     const PlaneTask tk = tasks[blockIdx.x];
     const MemberDev& md = mem[tk.member];
     if (!md.active) return;
     using S = typename PR::S;
+    using T = typename PR::T;
+    T* lds = lds_cells > 0 ? reinterpret_cast<T*>(batch_lds) : nullptr;
     const PlaneBlock pb = md.fg.blocks[tk.block];
     if (pb.plane < 0) return;
     const PlaneDesc& pd = md.fg.planes[pb.plane];
@@ -3095,14 +3095,14 @@ __global__ __launch_bounds__(kPlaneBlock) void k_batch_force_planes(const Member
         const S* Q = static_cast<const S*>(md.Q[0]);
         if (md.even)
             force_plane<PR, L, STREAM, MOMSRC_ESO_POPS, 0>(md.fg, pd, pb.rank, Q, md.F, prev, next, md.g, md.alfa, md.beta,
-                                                            md.rho0);
+                                                            md.rho0, lds, lds_cells);
         else
             force_plane<PR, L, STREAM, MOMSRC_ESO_POPS, 1>(md.fg, pd, pb.rank, Q, md.F, prev, next, md.g, md.alfa, md.beta,
-                                                            md.rho0);
+                                                            md.rho0, lds, lds_cells);
     } else {
         const S* Q = static_cast<const S*>(md.Q[1 - md.cur]);
         force_plane<PR, L, STREAM, MOMSRC_PULL_POPS, 1>(md.fg, pd, pb.rank, Q, md.F, prev, next, md.g, md.alfa, md.beta,
-                                                         md.rho0);
+                                                         md.rho0, lds, lds_cells);
     }
 }
 
@@ -3143,7 +3143,9 @@ hipError_t batch_step_t(const BatchPlan& p, hipStream_t s) {
                            p.off_words, n, p.tot_words);
     if (p.n_tasks > 0)
         hipLaunchKernelGGL((k_batch_force_planes<PR, L, STREAM>), dim3((uint32_t)p.n_tasks),
-                           dim3(p.force_bs > 0 ? p.force_bs : kPlaneBlock), 0, s, p.mem, p.tasks);
+                           dim3(p.force_bs > 0 ? p.force_bs : kPlaneBlock),
+                           (uint32_t)(p.force_lds_cells * 3 * sizeof(typename PR::T)), s, p.mem, p.tasks,
+                           p.force_lds_cells);
     hipLaunchKernelGGL(k_batch_advance<STREAM>, dim3(nblk(n, 64)), dim3(64), 0, s, p.mem_rw, n);
     return hipGetLastError();
 }
